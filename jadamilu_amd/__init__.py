@@ -1,0 +1,80 @@
+"""jadamilu_amd -- MI355X-native Jacobi-Davidson eigensolver behind the
+DPJD / DPJDREVCOM / DJDREVCOM calling sequence.
+
+The native library (libjadamilu_amd.so, built by `python -m jadamilu_amd.build`)
+holds the HIP kernels and the C-ABI of include/jadamilu_amd.h.  This package
+only loads it and offers thin Python mirrors of the reference drivers; there
+is no CPU fallback: every entry point raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+__all__ = ["lib", "LIB_PATH", "NativeLibraryMissing"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjadamilu_amd.so")
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+_i = C.POINTER(C.c_int)
+_l = C.POINTER(C.c_long)
+_d = C.POINTER(C.c_double)
+_v = C.c_void_p
+
+
+class JdhipStats(C.Structure):
+    _fields_ = [("inner_iters", C.c_long), ("host_syncs", C.c_long),
+                ("n_precond_setups", C.c_int), ("spmv_lanes", C.c_int),
+                ("setup_shifts", C.c_double * 64)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "dpjd_": (None, [_i, _d, _i, _i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _d, _d, _d, _d,
+                     _i, _i, _i, _d]),
+    "dpjdrevcom_": (None, [_i, _d, _i, _i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _d, _d, _d,
+                           _d, _i, _i, _i, _i, _i, _i, _d]),
+    "djdrevcom_": (None, [_i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _d, _i, _i, _i, _i, _i,
+                          _d]),
+    "dpjdcleanup_": (None, []),
+    "dpjd": (None, None), "dpjdrevcom": (None, None), "djdrevcom": (None, None),
+    "dpjdcleanup": (None, []),
+    "jdhip_create": (_v, [C.c_int]),
+    "jdhip_destroy": (None, [_v]),
+    "jdhip_pjd": (None, [_v, C.c_int, _d, _i, _i, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
+                         C.c_int, C.c_int, _i, C.c_double, _d, _d, _d, _i, C.c_int, _i, _d]),
+    "jdhip_pjdrevcom": (None, [_v, C.c_int, _d, _i, _i, _d, _d, _v, C.c_long, _i, C.c_double,
+                               C.c_int, C.c_int, C.c_int, _i, C.c_double, _d, _d, _d, _i, _i,
+                               _l, _l, C.c_int, _i, _d]),
+    "jdhip_jdrevcom": (None, [_v, C.c_int, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
+                              C.c_int, C.c_int, _i, C.c_double, _i, _l, _l, C.c_int, _i, _d]),
+    "jdhip_cleanup": (None, [_v]),
+    "jdhip_stream": (_v, [_v]),
+    "jdhip_set_matrix": (C.c_int, [_v, C.c_int, _i, _i, _d, C.c_int]),
+    "jdhip_spmv": (C.c_int, [_v, _v, _v]),
+    "jdhip_get_stats": (None, [_v, C.POINTER(JdhipStats)]),
+    "jdhip_last_error": (C.c_char_p, [_v]),
+}
+
+
+def lib():
+    """Load the native library (raises NativeLibraryMissing if not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} not found: build it with `python -m jadamilu_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            if args is not None:
+                f.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,682 @@
+// driver.cpp -- preconditioner manager and C-ABI entry points.
+//
+// Solver restates PJDRVCOM (src/jd/JDaux.F:5-544): it owns the preconditioner
+// (diagonal, DGLPRECSETUP/DGLPRECSOL, src/jd/dglprec*.c; multilevel ILDL in
+// ildl_*.cpp), applies the adaptive re-setup rules each time the JD engine
+// asks for a preconditioner solve, and routes matrix-vector products either
+// to the device SpMV (DPJD) or back to the caller (DPJDREVCOM/DJDREVCOM).
+// The COMMON blocks /PJDINITPJD/ and /JDRVCOMJD/ (JDaux.F:30-47) become
+// members of one Solver per context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/jadamilu_amd.h"
+#include "engine.hpp"
+#include "ildl.hpp"
+#include "kernels.hpp"
+
+namespace jdhip {
+
+struct Solver;
+
+struct SolverOps : Ops {
+    Solver *S = nullptr;
+    bool mv = false, pc = false;
+    bool has_matvec() const override { return mv; }
+    bool has_precond() const override { return pc; }
+    void matvec(const double *x, double *y, const double *eta_dev, double eta, const double *u,
+                double *udot) override;
+    void precond(JdEngine &e, double *z, const double *r) override;
+};
+
+struct Solver {
+    DevCtx ctx;
+    JdEngine eng{&ctx};
+    SolverOps ops;
+    std::string err;
+
+    // COMMON /PJDINITPJD/ (zero-initialised like the Fortran COMMON storage)
+    double timefact = 0, shift0 = 0, diagmin = 0, shiftmax = 0;
+    double slightlyless = 0.8, toldiv = 2.0;  // src/blaslike/common.f:14-15
+    int factdgl = 0, factspd = 0, prvdr = 0;
+    bool reenterfirsttime = false, flagsingle = false;
+    int nw = 0, ierr = 0;
+    double rprev = 0;
+
+    // diagonal preconditioner
+    double *d_diag = nullptr;
+    int ndiag = 0;
+    // multilevel ILDL preconditioner
+    std::unique_ptr<IldlPrecond> ildl;
+    // device operator for internal matvecs
+    CsrDev A;
+    int A_n = 0;
+    std::vector<int> A_host_ia;  // identity check for re-upload
+    // host-X mirror of the legacy API
+    double *dX = nullptr;
+    long dX_len = 0;
+    int pend_kind = 0;  // 1 matvec, 2 precond
+    long pend1 = 0, pend2 = 0;
+    // pointers of the current call (PJDRVCOM arguments)
+    int n = 0, isearch = 0;
+    double sigma = 0;
+    double *a = nullptr, *eigs = nullptr, *shift = nullptr, *droptol = nullptr, *mem = nullptr;
+    int *ja = nullptr, *ia = nullptr, *neig = nullptr, *icntl = nullptr;
+    std::vector<double> shifts_log;
+    int nsetups = 0;
+
+    explicit Solver(int dev) {
+        ctx.init(dev);
+        ops.S = this;
+    }
+    ~Solver() {
+        release_precond();
+        free_matrix();
+        if (dX) hipFree(dX);
+        ctx.release();
+    }
+    void free_matrix() {
+        if (A.rowptr) hipFree(A.rowptr);
+        if (A.col) hipFree(A.col);
+        if (A.val) hipFree(A.val);
+        A = CsrDev{};
+        A_n = 0;
+    }
+    void release_precond() {
+        if (d_diag) hipFree(d_diag);
+        d_diag = nullptr;
+        ndiag = 0;
+        ildl.reset();
+    }
+    void ensure_dx(long lx) {
+        if (dX_len >= lx) return;
+        if (dX) JDHIP_CHECK(hipFree(dX));
+        JDHIP_CHECK(hipMalloc(&dX, sizeof(double) * lx));
+        dX_len = lx;
+    }
+
+    void upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr);
+    int autotune_lpr();
+
+    // PJDINIT (JDaux.F:549-1325)
+    void pjdinit(int isearch_, double sigma_, double shift_in);
+    void dglprecsetup(int isearch_, double shift_in, double *shift0_out, double *diagmin_out);
+    void precond_request(double *z, const double *r);
+
+    void pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, double *res,
+                  double *x_dev, long lx, int *neig_, double sigma_, int isearch_, int ninit,
+                  int madspace, int *iter, double tol, double *shift_, double *droptol_,
+                  double *mem_, int *icntl_, int *ijob, long *ndx1, long *ndx2, int iprint,
+                  int *info, double *gap, bool internal_mv);
+};
+
+void SolverOps::matvec(const double *x, double *y, const double *eta_dev, double eta,
+                       const double *u, double *udot) {
+    spmv(S->ctx.stream, S->ctx.rb, S->A, x, y, eta_dev, eta, u, udot);
+}
+void SolverOps::precond(JdEngine &, double *z, const double *r) { S->precond_request(z, r); }
+
+// Full (both triangles) CSR of the symmetric matrix, built on the host from
+// the upper triangle and uploaded once: the SpMV then reads each row with
+// coalesced loads and needs no atomics.
+void Solver::upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr) {
+    free_matrix();
+    std::vector<int> cnt(n_ + 1, 0);
+    for (int i = 0; i < n_; ++i)
+        for (int k = ia_[i] - 1; k < ia_[i + 1] - 1; ++k) {
+            int j = ja_[k] - 1;
+            cnt[i + 1]++;
+            if (j != i) cnt[j + 1]++;
+        }
+    for (int i = 0; i < n_; ++i) cnt[i + 1] += cnt[i];
+    long nnz = cnt[n_];
+    std::vector<int> col(nnz), pos(cnt.begin(), cnt.end() - 1);
+    std::vector<double> val(nnz);
+    // lower-triangle entries first (ascending column), then the row's own
+    for (int i = 0; i < n_; ++i)
+        for (int k = ia_[i] - 1; k < ia_[i + 1] - 1; ++k) {
+            int j = ja_[k] - 1;
+            if (j != i) {
+                int p = pos[j]++;
+                col[p] = i;
+                val[p] = a_[k];
+            }
+        }
+    for (int i = 0; i < n_; ++i)
+        for (int k = ia_[i] - 1; k < ia_[i + 1] - 1; ++k) {
+            int p = pos[i]++;
+            col[p] = ja_[k] - 1;
+            val[p] = a_[k];
+        }
+    JDHIP_CHECK(hipMalloc(&A.rowptr, sizeof(int) * (n_ + 1)));
+    JDHIP_CHECK(hipMalloc(&A.col, sizeof(int) * std::max(nnz, 1L)));
+    JDHIP_CHECK(hipMalloc(&A.val, sizeof(double) * std::max(nnz, 1L)));
+    JDHIP_CHECK(hipMemcpy(A.rowptr, cnt.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    A.n = n_;
+    A.nnz = nnz;
+    A_n = n_;
+    A.lpr = lpr > 0 ? lpr : 8;
+    if (lpr <= 0) A.lpr = autotune_lpr();
+}
+
+// Pick lanes-per-row by timing the candidates on this matrix.
+int Solver::autotune_lpr() {
+    if (A.n < 4096) return 4;
+    double *x = nullptr, *y = nullptr;
+    JDHIP_CHECK(hipMalloc(&x, sizeof(double) * A.n));
+    JDHIP_CHECK(hipMalloc(&y, sizeof(double) * A.n));
+    fill(ctx.stream, x, 1.0, A.n);
+    hipEvent_t e0, e1;
+    JDHIP_CHECK(hipEventCreate(&e0));
+    JDHIP_CHECK(hipEventCreate(&e1));
+    int best = 8;
+    float bt = 1e30f;
+    for (int lpr : {1, 2, 4, 8, 16, 32}) {
+        A.lpr = lpr;
+        spmv(ctx.stream, ctx.rb, A, x, y, nullptr, 0.0, nullptr, nullptr);
+        JDHIP_CHECK(hipEventRecord(e0, ctx.stream));
+        for (int r = 0; r < 10; ++r) spmv(ctx.stream, ctx.rb, A, x, y, nullptr, 0.0, nullptr, nullptr);
+        JDHIP_CHECK(hipEventRecord(e1, ctx.stream));
+        JDHIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < bt) {
+            bt = ms;
+            best = lpr;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+    return best;
+}
+
+// DGLPRECSETUP (src/jd/dglprecsetup.c:11-82)
+void Solver::dglprecsetup(int isearch_, double shift_in, double *shift0_out,
+                          double *diagmin_out) {
+    std::vector<double> diag(n);
+    double dmin = a[0], ta = a[0];
+    for (int i = 1; i < n; ++i) {
+        dmin = std::min(dmin, a[i]);
+        ta = std::max(ta, a[i]);
+    }
+    double ati = std::max(dmin, -dmin), ata = std::max(ta, -ta);
+    ata = std::min(ata, ati);
+    double del = std::max(ta - dmin, ata);
+    if (isearch_ <= 1) {
+        *shift0_out = dmin - del / n;
+        factspd = -1;
+        if (isearch_ == 1 && shift_in < dmin) {
+            *shift0_out = shift_in;
+            factspd = 0;
+        }
+        for (int i = 0; i < n; ++i) diag[i] = 1.0 / (a[i] - *shift0_out);
+    } else {
+        factspd = 0;
+        del = 1. / del;
+        for (int i = 0; i < n; ++i) {
+            double t = a[i] - *shift0_out;
+            t = (t > 0.0) ? std::max(t, del) : std::min(t, -del);
+            diag[i] = 1.0 / t;
+        }
+    }
+    *diagmin_out = dmin;
+    if (!d_diag || ndiag != n) {
+        if (d_diag) JDHIP_CHECK(hipFree(d_diag));
+        JDHIP_CHECK(hipMalloc(&d_diag, sizeof(double) * n));
+        ndiag = n;
+    }
+    JDHIP_CHECK(hipMemcpyAsync(d_diag, diag.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                               ctx.stream));
+    JDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    factdgl = 1;
+}
+
+// PJDINIT (JDaux.F:549-1325)
+void Solver::pjdinit(int isearch_, double sigma_, double shift_in) {
+    if (factdgl != 0) {  // PJDCLEANUP1
+        release_precond();
+        factdgl = 0;
+    }
+    if (ja[0] < 0) {
+        // diagonal branch (JDaux.F:693-717); the COMMON diagmin is not
+        // updated here (the routine's value goes to a local copy)
+        if (isearch_ >= 2) shift0 = sigma_;
+        double rdiag = diagmin, s0 = shift0;
+        dglprecsetup(isearch_, shift_in, &s0, &rdiag);
+        shift0 = s0;
+        shifts_log.push_back(shift0);
+        ++nsetups;
+        return;
+    }
+    // multilevel ILDL branch
+    IldlOptions opt;
+    opt.isearch = isearch_;
+    opt.sigma = sigma_;
+    opt.shift = shift_in;
+    opt.droptol = droptol ? *droptol : 1e-3;
+    opt.mem = mem ? *mem : 0.0;
+    opt.icntl3 = icntl ? icntl[2] : 0;
+    opt.icntl4 = icntl ? icntl[3] : 0;
+    opt.single = icntl ? icntl[4] != 0 : false;
+    ildl = std::make_unique<IldlPrecond>(&ctx);
+    IldlResult r = ildl->setup(n, ia, ja, a, opt);
+    if (r.ierr != 0) {
+        ierr = r.ierr;
+        ildl.reset();
+        return;
+    }
+    shift0 = r.shift;
+    diagmin = r.diagmin;
+    shiftmax = r.shiftmax;
+    factspd = r.factspd;
+    if (droptol) *droptol = r.droptol;
+    timefact = r.time;
+    factdgl = -1;
+    shifts_log.push_back(shift0);
+    ++nsetups;
+}
+
+// IJOB=2 branch of PJDRVCOM (JDaux.F:282-471)
+void Solver::precond_request(double *z, const double *r) {
+    bool reinit = false;
+    int factspd0 = 0;
+    double tent = 0.0;
+    if (factdgl > 0 && !eng.flagcg && isearch <= 1 && (icntl[1] == 0 || icntl[1] == -2) &&
+        icntl[0] != 2) {
+        if (*neig >= 1) {
+            tent = eigs[nw - 1];
+            factspd0 = -1;
+        } else {
+            tent = eng.shift1 - eng.rcurr;
+            factspd0 = -1;
+            if (10 * eng.rcurr >= diagmin - tent && eng.shift1 > shift0) factspd0 = factspd;
+        }
+        if (diagmin > tent && factspd0 < 0 &&
+            10 * std::fabs(shift0 - tent) >= (diagmin - shift0)) {
+            *shift = tent;
+            reinit = true;
+        }
+    }
+    if (reinit) {
+        pjdinit(std::max(isearch, 1), sigma, *shift);
+        if (ierr != 0) throw std::runtime_error("preconditioner setup failed");
+        if (factspd0 < 0 && factspd <= 0) factspd = factspd0;
+    }
+    if (factdgl > 0) {
+        diag_apply(ctx.stream, z, d_diag, r, n);
+    } else if (ildl) {
+        ildl->apply(z, r);
+    } else {
+        throw std::runtime_error("no preconditioner");
+    }
+}
+
+void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, double *res,
+                      double *x_dev, long lx, int *neig_, double sigma_, int isearch_,
+                      int ninit, int madspace, int *iter, double tol, double *shift_,
+                      double *droptol_, double *mem_, int *icntl_, int *ijob, long *ndx1,
+                      long *ndx2, int iprint, int *info, double *gap, bool internal_mv) {
+    n = n_;
+    a = a_;
+    ja = ja_;
+    ia = ia_;
+    eigs = eigs_;
+    neig = neig_;
+    sigma = sigma_;
+    isearch = isearch_;
+    shift = shift_;
+    droptol = droptol_;
+    mem = mem_;
+    icntl = icntl_;
+    if (*ijob == 0) {
+        flagsingle = icntl[4] != 0;
+        ierr = 0;
+        shifts_log.clear();
+        nsetups = 0;
+        if ((icntl[1] != -1 && icntl[1] != -2) || factdgl == 0) {
+            prvdr = 0;
+            timefact = -1.0;
+            pjdinit(isearch, sigma, *shift);
+            reenterfirsttime = false;
+        } else {
+            prvdr = 0;
+            reenterfirsttime = true;
+            eng.rinit = eng.rcurr = eng.rcum = 0.0;
+            if (ja[0] < 0 && isearch >= 2) shift0 = sigma;
+        }
+        if (ierr != 0) {
+            *info = -54321;
+            *ijob = 0;
+            return;
+        }
+        rprev = 1.0;
+        nw = *neig;
+        ops.mv = internal_mv;
+        ops.pc = true;
+        eng.set_ops(&ops);
+    }
+    eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
+             ndx1, ndx2, iprint, info, gap);
+    if (*ijob == 3) *ijob = 1;
+    if (*ijob == 0 && *info > -54321 + 0 && ierr == 0) {
+        // JDaux.F:476-482: suggested SHIFT for a restart
+        *shift = shift0;
+        if (factdgl >= 0 && *neig > 0) *shift = eigs[0];
+    }
+}
+
+}  // namespace jdhip
+
+using jdhip::Solver;
+
+struct jdhip_ctx {
+    std::unique_ptr<Solver> S;
+    std::string err;
+};
+
+static void set_err(jdhip_ctx *c, const char *msg) {
+    if (c) c->err = msg;
+    std::fprintf(stderr, "jadamilu_amd: %s\n", msg);
+}
+
+extern "C" {
+
+jdhip_ctx *jdhip_create(int device) {
+    auto *c = new jdhip_ctx();
+    try {
+        c->S = std::make_unique<Solver>(device);
+    } catch (const std::exception &e) {
+        set_err(c, e.what());
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void jdhip_destroy(jdhip_ctx *ctx) { delete ctx; }
+
+void *jdhip_stream(jdhip_ctx *ctx) { return ctx ? (void *)ctx->S->ctx.stream : nullptr; }
+
+const char *jdhip_last_error(jdhip_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out) {
+    std::memset(out, 0, sizeof(*out));
+    if (!ctx) return;
+    Solver &S = *ctx->S;
+    out->inner_iters = S.eng.inner_iters;
+    out->host_syncs = S.eng.nsync;
+    out->n_precond_setups = S.nsetups;
+    out->spmv_lanes = S.A.lpr;
+    for (size_t i = 0; i < S.shifts_log.size() && i < 64; ++i)
+        out->setup_shifts[i] = S.shifts_log[i];
+}
+
+int jdhip_set_matrix(jdhip_ctx *ctx, int n, const int *ia, const int *ja, const double *a,
+                     int lanes_per_row) {
+    try {
+        ctx->S->upload_matrix(n, ia, ja, a, lanes_per_row);
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -1;
+    }
+}
+
+int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev) {
+    try {
+        Solver &S = *ctx->S;
+        if (!S.A.rowptr) throw std::runtime_error("jdhip_spmv: no matrix set");
+        jdhip::spmv(S.ctx.stream, S.ctx.rb, S.A, x_dev, y_dev, nullptr, 0.0, nullptr, nullptr);
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -1;
+    }
+}
+
+void jdhip_pjdrevcom(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *eigs,
+                     double *res, double *x_dev, long lx, int *neig, double sigma, int isearch,
+                     int ninit, int madspace, int *iter, double tol, double *shift,
+                     double *droptol, double *mem, int *icntl, int *ijob, long *ndx1,
+                     long *ndx2, int iprint, int *info, double *gap) {
+    try {
+        long l1 = *ndx1 - 1, l2 = *ndx2 - 1;
+        ctx->S->pjdrvcom(n, a, ja, ia, eigs, res, x_dev, lx, neig, sigma, isearch, ninit,
+                         madspace, iter, tol, shift, droptol, mem, icntl, ijob, &l1, &l2,
+                         iprint, info, gap, false);
+        *ndx1 = l1 + 1;
+        *ndx2 = l2 + 1;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+    }
+}
+
+void jdhip_jdrevcom(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_dev, long lx,
+                    int *neig, double sigma, int isearch, int ninit, int madspace, int *iter,
+                    double tol, int *ijob, long *ndx1, long *ndx2, int iprint, int *info,
+                    double *gap) {
+    try {
+        Solver &S = *ctx->S;
+        long l1 = *ndx1 - 1, l2 = *ndx2 - 1;
+        if (*ijob == 0) S.eng.set_ops(nullptr);
+        S.eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol,
+                   ijob, &l1, &l2, iprint, info, gap);
+        if (*ijob == 3) *ijob = 1;
+        *ndx1 = l1 + 1;
+        *ndx2 = l2 + 1;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+    }
+}
+
+void jdhip_pjd(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *eigs, double *res,
+               double *x_dev, long lx, int *neig, double sigma, int isearch, int ninit,
+               int madspace, int *iter, double tol, double *shift, double *droptol,
+               double *mem, int *icntl, int iprint, int *info, double *gap) {
+    try {
+        Solver &S = *ctx->S;
+        // JDaux.F:215-229: the diagonal entry moves to the front of each row
+        if (ja[0] >= 0) {
+            for (int i = 0; i < n; ++i)
+                for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+                    if (ja[k] == i + 1) {
+                        int l = ia[i] - 1;
+                        std::swap(ja[k], ja[l]);
+                        std::swap(a[k], a[l]);
+                    }
+            S.upload_matrix(n, ia, ja, a, 0);
+        }
+        int keep = icntl[0];
+        icntl[0] = 1;
+        int ijob = 0;
+        long l1 = 0, l2 = 0;
+        S.pjdrvcom(n, a, ja, ia, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace,
+                   iter, tol, shift, droptol, mem, icntl, &ijob, &l1, &l2, iprint, info, gap,
+                   true);
+        icntl[0] = keep;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+    }
+}
+
+void jdhip_cleanup(jdhip_ctx *ctx) {
+    if (!ctx) return;
+    ctx->S->release_precond();
+    ctx->S->factdgl = 0;
+}
+
+}  // extern "C"
+
+// ---- legacy symbols: host X mirrored on the device ------------------------
+static jdhip_ctx *g_ctx = nullptr;
+static std::mutex g_mu;
+
+static jdhip_ctx *legacy_ctx() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx) {
+        int dev = 0;
+        if (const char *e = std::getenv("JDHIP_DEVICE")) dev = std::atoi(e);
+        g_ctx = jdhip_create(dev);
+    }
+    return g_ctx;
+}
+
+static void copy_d2h(Solver &S, double *hx, long off, long cnt) {
+    JDHIP_CHECK(hipMemcpyAsync(hx + off, S.dX + off, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                               S.ctx.stream));
+    JDHIP_CHECK(hipStreamSynchronize(S.ctx.stream));
+}
+static void copy_h2d(Solver &S, const double *hx, long off, long cnt) {
+    JDHIP_CHECK(hipMemcpyAsync(S.dX + off, hx + off, sizeof(double) * cnt, hipMemcpyHostToDevice,
+                               S.ctx.stream));
+    JDHIP_CHECK(hipStreamSynchronize(S.ctx.stream));
+}
+
+// common revcom wrapper for host X: stages the requested slices
+template <typename Fn>
+static void legacy_revcom(int n, double *x, int lx, int ninit, int nwant, int *ijob, int *ndx1,
+                          int *ndx2, int *info, int *neig, Fn &&call) {
+    jdhip_ctx *c = legacy_ctx();
+    if (!c) {
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+        return;
+    }
+    Solver &S = *c->S;
+    try {
+        if (*ijob == 0) {
+            S.ensure_dx(lx);
+            if (ninit > 0) copy_h2d(S, x, 0, std::min<long>((long)ninit * n, lx));
+            S.pend_kind = 0;
+        } else if (S.pend_kind == 1) {
+            copy_h2d(S, x, S.pend2, n);  // caller wrote A*x into X(NDX2)
+        } else if (S.pend_kind == 2) {
+            copy_h2d(S, x, S.pend1, n);  // caller wrote P^-1 x into X(NDX1)
+        }
+        long l1 = 0, l2 = 0;
+        call(c, S.dX, &l1, &l2);
+        if (*ijob == 1 || *ijob == 2) {
+            S.pend_kind = *ijob;
+            S.pend1 = l1 - 1;
+            S.pend2 = l2 - 1;
+            long src = (*ijob == 1) ? S.pend1 : S.pend2;
+            copy_d2h(S, x, src, n);
+            *ndx1 = (int)l1;
+            *ndx2 = (int)l2;
+        } else {
+            S.pend_kind = 0;
+            long cnt = (long)n * std::max(1, std::min(nwant, *neig + 1));
+            if (*info >= 0 || *neig > 0) copy_d2h(S, x, 0, std::min<long>(cnt, lx));
+        }
+    } catch (const std::exception &e) {
+        set_err(c, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+    }
+}
+
+extern "C" {
+
+void dpjdrevcom_(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x,
+                 int *lx, int *neig, double *sigma, int *isearch, int *ninit, int *madspace,
+                 int *iter, double *tol, double *shift, double *droptol, double *mem,
+                 int *icntl, int *ijob, int *ndx1, int *ndx2, int *iprint, int *info,
+                 double *gap) {
+    static int nwant = 0;
+    if (*ijob == 0) nwant = *neig;
+    legacy_revcom(*n, x, *lx, *ninit, nwant, ijob, ndx1, ndx2, info, neig,
+                  [&](jdhip_ctx *c, double *dx, long *l1, long *l2) {
+                      *l1 = *ndx1;
+                      *l2 = *ndx2;
+                      jdhip_pjdrevcom(c, *n, a, ja, ia, eigs, res, dx, *lx, neig, *sigma,
+                                      *isearch, *ninit, *madspace, iter, *tol, shift, droptol,
+                                      mem, icntl, ijob, l1, l2, *iprint, info, gap);
+                  });
+}
+
+void djdrevcom_(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                double *sigma, int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                int *ijob, int *ndx1, int *ndx2, int *iprint, int *info, double *gap) {
+    static int nwant = 0;
+    if (*ijob == 0) nwant = *neig;
+    legacy_revcom(*n, x, *lx, *ninit, nwant, ijob, ndx1, ndx2, info, neig,
+                  [&](jdhip_ctx *c, double *dx, long *l1, long *l2) {
+                      *l1 = *ndx1;
+                      *l2 = *ndx2;
+                      jdhip_jdrevcom(c, *n, eigs, res, dx, *lx, neig, *sigma, *isearch,
+                                     *ninit, *madspace, iter, *tol, ijob, l1, l2, *iprint,
+                                     info, gap);
+                  });
+}
+
+void dpjd_(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x, int *lx,
+           int *neig, double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+           double *tol, double *shift, double *droptol, double *mem, int *icntl, int *iprint,
+           int *info, double *gap) {
+    jdhip_ctx *c = legacy_ctx();
+    if (!c) {
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        return;
+    }
+    Solver &S = *c->S;
+    int nwant = *neig;
+    try {
+        S.ensure_dx(*lx);
+        if (*ninit > 0) copy_h2d(S, x, 0, std::min<long>((long)*ninit * *n, *lx));
+        jdhip_pjd(c, *n, a, ja, ia, eigs, res, S.dX, *lx, neig, *sigma, *isearch, *ninit,
+                  *madspace, iter, *tol, shift, droptol, mem, icntl, *iprint, info, gap);
+        long cnt = (long)*n * std::max(1, std::min(nwant, *neig + 1));
+        copy_d2h(S, x, 0, std::min<long>(cnt, *lx));
+    } catch (const std::exception &e) {
+        set_err(c, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+    }
+}
+
+void dpjdcleanup_(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_ctx) jdhip_cleanup(g_ctx);
+}
+
+void dpjd(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x, int *lx,
+          int *neig, double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+          double *tol, double *shift, double *droptol, double *mem, int *icntl, int *iprint,
+          int *info, double *gap) {
+    dpjd_(n, a, ja, ia, eigs, res, x, lx, neig, sigma, isearch, ninit, madspace, iter, tol,
+          shift, droptol, mem, icntl, iprint, info, gap);
+}
+void dpjdrevcom(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x,
+                int *lx, int *neig, double *sigma, int *isearch, int *ninit, int *madspace,
+                int *iter, double *tol, double *shift, double *droptol, double *mem,
+                int *icntl, int *ijob, int *ndx1, int *ndx2, int *iprint, int *info,
+                double *gap) {
+    dpjdrevcom_(n, a, ja, ia, eigs, res, x, lx, neig, sigma, isearch, ninit, madspace, iter,
+                tol, shift, droptol, mem, icntl, ijob, ndx1, ndx2, iprint, info, gap);
+}
+void djdrevcom(int *n, double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+               int *isearch, int *ninit, int *madspace, int *iter, double *tol, int *ijob,
+               int *ndx1, int *ndx2, int *iprint, int *info, double *gap) {
+    djdrevcom_(n, eigs, res, x, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
+               ndx1, ndx2, iprint, info, gap);
+}
+void dpjdcleanup(void) { dpjdcleanup_(); }
+
+}  // extern "C"

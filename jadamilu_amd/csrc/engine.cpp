@@ -1,0 +1,1007 @@
+// engine.cpp -- device-resident Jacobi-Davidson engine (see engine.hpp).
+//
+// Step-by-step restatement of JDRVCOM (src/jd/JDaux.F:1391-2415).  Labels
+// L<n> carry the reference's statement labels so each block can be checked
+// against the Fortran.  n-length operations are HIP kernels on ctx->stream;
+// the projected J x J problems run on the host.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "dense.hpp"
+
+namespace jdhip {
+
+static constexpr double kEps = DBL_EPSILON * 0.5;  // dlamch('E')
+static constexpr double kInf = DBL_MAX;            // dlamch('O')
+
+// ---------------------------------------------------------------------------
+void DevCtx::init(int dev) {
+    device = dev;
+    JDHIP_CHECK(hipSetDevice(dev));
+    JDHIP_CHECK(hipStreamCreate(&stream));
+    JDHIP_CHECK(hipMalloc(&rb.partials, sizeof(double) * kMaxRedBlocks * kMaxCols));
+    JDHIP_CHECK(hipMalloc(&rb.counter, sizeof(unsigned) * 16));
+    JDHIP_CHECK(hipMemset(rb.counter, 0, sizeof(unsigned) * 16));
+    JDHIP_CHECK(hipMalloc(&d_small, sizeof(double) * 8192));
+    JDHIP_CHECK(hipMemset(d_small, 0, sizeof(double) * 8192));
+    rb.out = d_small;
+    JDHIP_CHECK(hipMalloc(&d_sc, sizeof(InnerScalars)));
+    JDHIP_CHECK(hipMemset(d_sc, 0, sizeof(InnerScalars)));
+    JDHIP_CHECK(hipHostMalloc(&h_pin, sizeof(double) * 8192, hipHostMallocDefault));
+    JDHIP_CHECK(hipHostMalloc(&h_sc, sizeof(InnerScalars), hipHostMallocDefault));
+    std::memset(h_sc, 0, sizeof(InnerScalars));
+}
+
+void DevCtx::release() {
+    if (!stream) return;
+    hipSetDevice(device);
+    hipStreamSynchronize(stream);
+    hipFree(rb.partials);
+    hipFree(rb.counter);
+    hipFree(d_small);
+    hipFree(d_sc);
+    hipFree(d_tmp);
+    hipHostFree(h_pin);
+    hipHostFree(h_sc);
+    hipStreamDestroy(stream);
+    stream = nullptr;
+    d_tmp = nullptr;
+    tmp_len = 0;
+}
+
+void DevCtx::ensure_tmp(long len) {
+    if (len <= tmp_len) return;
+    if (d_tmp) {
+        JDHIP_CHECK(hipStreamSynchronize(stream));
+        JDHIP_CHECK(hipFree(d_tmp));
+    }
+    JDHIP_CHECK(hipMalloc(&d_tmp, sizeof(double) * len));
+    tmp_len = len;
+}
+
+// ---------------------------------------------------------------------------
+void JdEngine::fetch(double *host, const double *dev, long cnt) {
+    if (cnt <= 0) return;
+    JDHIP_CHECK(hipMemcpyAsync(ctx_->h_pin, dev, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                               ctx_->stream));
+    ctx_->sync();
+    ++nsync;
+    std::memcpy(host, ctx_->h_pin, sizeof(double) * cnt);
+}
+
+void JdEngine::put(double *dev, const double *host, long cnt) {
+    if (cnt <= 0) return;
+    std::memcpy(ctx_->h_pin + 4096, host, sizeof(double) * cnt);
+    JDHIP_CHECK(hipMemcpyAsync(dev, ctx_->h_pin + 4096, sizeof(double) * cnt,
+                               hipMemcpyHostToDevice, ctx_->stream));
+    ctx_->sync();
+}
+
+double JdEngine::dnorm(const double *w, long n) {
+    double *d = ctx_->slot(2);
+    dot(ctx_->stream, ctx_->rb, w, w, n, d);
+    double v;
+    fetch(&v, d, 1);
+    return std::sqrt(v);
+}
+
+// REPGS (JDaux.F:2674-2753): iterated classical Gram-Schmidt of w against the
+// jv columns at voff.  y[0:jv) coefficients, y[jv] final norm (gam >= 0).
+void JdEngine::repgs(int n, long voff, int jv, long woff, std::vector<double> &y, int gam) {
+    if (jv <= 0) return;
+    if ((int)y.size() < 2 * jv + 2) y.resize(2 * jv + 2);
+    hipStream_t s = ctx_->stream;
+    const double *V = X_ + voff;
+    double *w = X_ + woff;
+    double *dy = ctx_->slot(0);
+    if (gam < 0) {
+        multidot(s, ctx_->rb, V, n, jv, w, n, dy);
+        gemv_n(s, ctx_->rb, w, V, n, jv, dy, -1.0, 1.0, n, nullptr);
+        fetch(y.data(), dy, jv);
+        return;
+    }
+    bool nrnd = true;
+restart : {
+    double nrn = dnorm(w, n), nro;
+    double nr = kEps * nrn;
+    if (nrnd) std::fill(y.begin(), y.begin() + jv, 0.0);
+    for (;;) {
+        nro = nrn;
+        multidot(s, ctx_->rb, V, n, jv, w, n, dy);
+        gemv_n(s, ctx_->rb, w, V, n, jv, dy, -1.0, 1.0, n, ctx_->slot(1));
+        double buf[kMaxCols + 1];
+        fetch(buf, dy, kMaxCols + 1);
+        nrn = std::sqrt(buf[kMaxCols]);
+        if (nrnd)
+            for (int c = 0; c < jv; ++c) y[c] += buf[c];
+        if (nrn <= nr) goto fail;
+        if (!(nrn < 0.5 * nro)) break;
+    }
+    y[jv] = nrn;
+    return;
+}
+fail:
+    if (gam == 1) {
+        if (jv < n) {
+            // breakdown: expand with a pseudo-random vector (JDaux.F:2741-2745)
+            std::vector<double> r(n);
+            for (int i = 0; i < n; ++i) {
+                rng_ ^= rng_ >> 12;
+                rng_ ^= rng_ << 25;
+                rng_ ^= rng_ >> 27;
+                r[i] = (double)((rng_ * 2685821657736338717ULL) >> 11) * 0x1.0p-53;
+            }
+            JDHIP_CHECK(hipMemcpy(w, r.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            nrnd = false;
+            goto restart;
+        }
+    } else {
+        fill(s, w, 0.0, n);
+        y[jv] = 0.0;
+    }
+}
+
+// SORTEV (JDaux.F:2464-2540): the permutation is computed on the host, the
+// column cycles are applied on the device.
+void JdEngine::sortev_dev(int n, int j, long voff, double *eigs, double *r, int it,
+                          double sigma) {
+    if (j <= 1) return;
+    std::vector<double> key(j);
+    std::vector<int> iw(j);
+    for (int i = 0; i < j; ++i) {
+        if (it == 2 || it == -2) key[i] = -std::fabs(eigs[i] - sigma);
+        else if (it == 4 || it == -4) key[i] = eigs[i];
+        else key[i] = -eigs[i];
+    }
+    sortrx(j, key.data(), iw.data());
+    if (it == -2) {
+        int is = iw[j - 1];
+        for (int i = j - 2; i >= 0; --i) iw[i + 1] = iw[i];
+        iw[0] = is;
+    }
+    ctx_->ensure_tmp((long)n);
+    hipStream_t s = ctx_->stream;
+    const size_t bytes = sizeof(double) * n;
+    auto col = [&](int k) { return X_ + voff + (long)(k - 1) * n; };
+    int ic = 0;
+    for (;;) {
+        int is = 0;
+        for (;;) {
+            ++is;
+            if (iw[is - 1] < 0) continue;
+            if (iw[is - 1] == is) {
+                iw[is - 1] = -iw[is - 1];
+                if (++ic >= j) return;
+                continue;
+            }
+            break;
+        }
+        int ir = iw[is - 1];
+        JDHIP_CHECK(hipMemcpyAsync(ctx_->d_tmp, col(is), bytes, hipMemcpyDeviceToDevice, s));
+        double r1 = 0.0, w1 = eigs[is - 1];
+        if (it == 4 || it == -4) r1 = r[is - 1];
+        for (;;) {
+            JDHIP_CHECK(hipMemcpyAsync(col(is), col(ir), bytes, hipMemcpyDeviceToDevice, s));
+            if (it == 4 || it == -4) r[is - 1] = r[ir - 1];
+            eigs[is - 1] = eigs[ir - 1];
+            iw[is - 1] = -ir;
+            ++ic;
+            is = ir;
+            ir = iw[is - 1];
+            if (!(iw[ir - 1] > 0)) break;
+        }
+        JDHIP_CHECK(hipMemcpyAsync(col(is), ctx_->d_tmp, bytes, hipMemcpyDeviceToDevice, s));
+        if (it == 4 || it == -4) r[is - 1] = r1;
+        eigs[is - 1] = w1;
+        iw[is - 1] = -ir;
+        if (++ic >= j) return;
+    }
+}
+
+// SOLPROJEP (JDaux.F:3271-3358), standard problem, on host matrices.
+int JdEngine::solprojep(int j, double *mm, double *nn, double *th, double *th2,
+                        std::vector<double> &w, int is) {
+    if ((int)w.size() < std::max(j, 2)) w.resize(std::max(j, 2), 0.0);
+    int info;
+    if (is == 2 || is == -2) {
+        info = sym_gen_eig(j, mm, modspace, nn, modspace, w.data());
+        if (info) return info;
+        // SORTEV on the small eigenvector matrix (JDaux.F:3350-3352)
+        if (j > 1) {
+            std::vector<double> key(j);
+            std::vector<int> iw(j);
+            for (int i = 0; i < j; ++i) key[i] = -std::fabs(w[i]);
+            sortrx(j, key.data(), iw.data());
+            if (is == -2) {
+                int t = iw[j - 1];
+                for (int i = j - 2; i >= 0; --i) iw[i + 1] = iw[i];
+                iw[0] = t;
+            }
+            std::vector<double> m2((size_t)modspace * j), w2(j);
+            for (int c = 0; c < j; ++c) {
+                w2[c] = w[iw[c] - 1];
+                std::memcpy(&m2[(size_t)c * modspace], mm + (long)(iw[c] - 1) * modspace,
+                            sizeof(double) * j);
+            }
+            for (int c = 0; c < j; ++c) {
+                w[c] = w2[c];
+                std::memcpy(mm + (long)c * modspace, &m2[(size_t)c * modspace],
+                            sizeof(double) * j);
+            }
+        }
+        *th2 = w[1];
+        return 0;
+    }
+    info = sym_eig(j, mm, modspace, w.data());
+    if (info) return info;
+    if (is == 3) {
+        for (int i = 0; i < j; ++i) nn[i] = w[i];
+    } else {
+        *th = w[0];
+        *th2 = w[1];
+    }
+    return 0;
+}
+
+// SHRINK (JDaux.F:2418-2462): QR of the k x j Ritz block at MP(:, ucol_off..),
+// [V W] := [V W] Q(:,1:j) on the device, MM (and NN) := Q^T . Q on the host.
+int JdEngine::shrink(int n, int j, int k, int ucol_off, int iserch) {
+    double *U = MP.data() + ucol_off;
+    std::vector<double> tau(j), q((size_t)k * j);
+    house_qr(k, j, U, modspace, tau.data());
+    form_q(k, j, j, U, modspace, tau.data(), q.data());
+    double *dq = ctx_->d_small + 4096;
+    put(dq, q.data(), (long)k * j);
+    ctx_->ensure_tmp((long)n * j);
+    hipStream_t s = ctx_->stream;
+    tall_gemm(s, X_ + vv, n, k, dq, k, j, ctx_->d_tmp, n, n);
+    JDHIP_CHECK(hipMemcpyAsync(X_ + vv, ctx_->d_tmp, sizeof(double) * n * j,
+                               hipMemcpyDeviceToDevice, s));
+    tall_gemm(s, X_ + ww, n, k, dq, k, j, ctx_->d_tmp, n, n);
+    JDHIP_CHECK(hipMemcpyAsync(X_ + ww, ctx_->d_tmp, sizeof(double) * n * j,
+                               hipMemcpyDeviceToDevice, s));
+    const int ld = modspace;
+    for (int a = 1; a < k; ++a)
+        for (int b = 0; b < a; ++b) MM[a + b * ld] = MM[b + a * ld];
+    apply_q_right(k, k, j, U, ld, tau.data(), MM.data(), ld);
+    apply_qt_left(k, k, j, U, ld, tau.data(), MM.data(), ld);
+    if (iserch == 2) {
+        for (int a = 1; a < k; ++a)
+            for (int b = 0; b < a; ++b) NN[a + b * ld] = NN[b + a * ld];
+        apply_q_right(k, k, j, U, ld, tau.data(), NN.data(), ld);
+        apply_qt_left(k, k, j, U, ld, tau.data(), NN.data(), ld);
+    }
+    return 0;
+}
+
+// Initial projected matrices (JDaux.F:1646-1660).
+void JdEngine::gram_initial(int n) {
+    hipStream_t s = ctx_->stream;
+    const int ld = modspace;
+    double a[kMaxCols], b[kMaxCols];
+    if (iserch == 2) {
+        // W := W - sigma V over the J initial columns (contiguous)
+        // (sigma folded into axpyalpha by the caller)
+        axpy(s, X_ + ww, axpyalpha, X_ + bv, (long)n * J);
+        for (int c = 0; c < J; ++c) {
+            multidot(s, ctx_->rb, X_ + ww, n, c + 1, X_ + ww + (long)c * n, n, ctx_->slot(0));
+            fetch(a, ctx_->slot(0), c + 1);
+            for (int r = 0; r <= c; ++r) NN[r + c * ld] = a[r];
+        }
+    }
+    for (int c = 0; c < J; ++c) {
+        multidot(s, ctx_->rb, X_ + vv, n, c + 1, X_ + ww + (long)c * n, n, ctx_->slot(0));
+        multidot(s, ctx_->rb, X_ + ww, n, c + 1, X_ + vv + (long)c * n, n, ctx_->slot(1));
+        double buf[2 * kMaxCols];
+        fetch(buf, ctx_->slot(0), 2 * kMaxCols);
+        for (int r = 0; r <= c; ++r) {
+            a[r] = buf[r];
+            b[r] = buf[kMaxCols + r];
+            MM[r + c * ld] = a[r] + b[r];
+        }
+    }
+    for (auto &v : MM) v *= 0.5;
+}
+
+// STOPTEST (JDaux.F:2542-2672), standard problem; xn2 = ||x||^2 (device).
+void JdEngine::stoptest(int n, double tol, double *rk, double alprho_, double gk_,
+                        double *theta_, double eta_, int *info, double trs_, double xn2) {
+    (void)n;
+    if (*info == 0) goto L10;
+    st_istag = 0;
+    st_istt = 0;
+    st_g0 = *rk;
+    st_gkm1 = *rk;
+    st_gtrs = trs_ * st_g0;
+    if (st_gtrs < tol) st_gtrs = tol;
+    st_gs = st_gtrs;
+    st_gkmin = st_g0;
+    st_bk = *theta_ - eta_;
+    st_un2 = -1.0;
+L10:
+    st_bk = st_bk - alprho_;
+    st_beta = std::fabs(st_bk);
+    if (gk_ < st_gkmin) {
+        st_gkmin = gk_;
+        st_istag = 0;
+    }
+    if (gk_ > st_gkm1) {
+        ++st_istag;
+        ++st_istt;
+    }
+    if (st_istag >= 25 || st_istt >= 100) goto L40;
+    if (st_gkmin >= st_gtrs) goto L20;
+    st_toupdate = true;
+    if (gk_ >= st_gs) goto L15;
+L13:
+    st_alp = 1.0;
+    st_tn = std::sqrt(xn2);
+    st_un2 = 1.0 + st_tn * st_tn;
+    st_toupdate = false;
+    st_gs = (gk_ < 0.1 * st_g0) ? tol : 0.1 * st_g0;
+    if (st_gs < tol) st_gs = tol;
+L15:
+    *rk = std::sqrt(gk_ * gk_ * st_un2 + st_tn * st_tn * st_beta * st_beta) / st_un2;
+    if (*rk < tol && st_toupdate) goto L13;
+    if (*rk < tol) goto L40;
+    if (gk_ * std::sqrt(st_un2) < 15 * st_tn * st_beta &&
+        st_beta * st_tn > 0.5 * tol * st_un2 * st_alp)
+        goto L40;
+    if (st_istag >= 3) goto L40;
+    if (st_istag >= 1 && st_beta * st_tn > 0.5 * tol * st_un2 * st_alp && gk_ < st_gtrs)
+        goto L40;
+L20:
+    *info = 0;
+    st_gkm1 = gk_;
+    return;
+L40:
+    *info = std::max(*info, 1);
+    *theta_ = st_bk;
+}
+
+// ---------------------------------------------------------------------------
+void JdEngine::step(int n, double *eigs, double *res, double *x, long lx, int *neig,
+                    double sigma, int isearch, int ninit, int madspace, int *iter, double tol,
+                    int *ijob, long *ndx1, long *ndx2, int iprint, int *info, double *gap) {
+    (void)iprint;
+    X_ = x;
+    hipStream_t s = ctx_->stream;
+    const RedBuf &rb = ctx_->rb;
+    const size_t nb = sizeof(double) * (size_t)n;
+    auto D = [&](long off) { return X_ + off; };
+
+    if (*ijob == 0) goto L1;
+    switch (rlbl) {
+    case 1: goto L5;
+    case 2: goto L225;
+    case 3: goto L37;
+    case 4: goto L38;
+    case 5: goto L255;
+    case 6: goto L55;
+    default: break;
+    }
+    *info = -15;
+    goto L190;
+
+L1:
+    flagcg = true;
+    neweig = true;
+    rcum = 0.0;
+    rinitm = 1.0;
+    inner_iters = 0;
+    nsync = 0;
+    *info = 0;
+    if (n < 2) *info = -1;
+    else if (*neig <= 0) *info = -5;
+    else if (madspace <= 2) *info = -9;
+    else if ((long)n * (2 * madspace + *neig + 4) + 3L * madspace * madspace +
+                 std::max(madspace * madspace, *neig) > lx)
+        *info = -10;
+    else if (*iter <= 0) *info = -11;
+    else if (tol <= 0.0) *info = -12;
+    else if (std::min(*neig, n) + std::min(madspace, n) > kMaxCols)
+        *info = -9;  // tall-skinny kernels hold at most kMaxCols columns
+    if (*info != 0) goto L190;
+    modspace = std::min(madspace, n);
+    nwant = std::min(*neig, n);
+    mvmax = *iter;
+    iserch = std::min(isearch, 2);  // JDaux.F:1530-1531 (second assignment wins)
+    {
+        const long N = n, MS = modspace;
+        xx = 0;
+        vv = xx + (long)nwant * N;
+        ww = vv + MS * N;
+        bx = xx;
+        bv = vv;
+        // MM, NN, MP, NP live on the host; WCG keeps the reference offset
+        long mm_off = ww + MS * N;
+        wcg = mm_off + 3 * MS * MS + std::max(MS * MS, (long)nwant);
+        xcg2 = ww + (MS - 1) * N;
+        reig = wcg;
+        uu = xx + (long)(nwant - 1) * N;
+        bu = bx + (long)(nwant - 1) * N;
+        kxp = vv;
+        kbxp = bv;
+    }
+    MM.assign((size_t)modspace * modspace, 0.0);
+    NN.assign((size_t)modspace * modspace, 0.0);
+    MP.assign((size_t)modspace * modspace, 0.0);
+    NP.assign((size_t)modspace * modspace, 0.0);
+    WL.assign(3 * modspace + 4, 0.0);
+    WL2.assign(modspace + 2, 0.0);
+    REIGW.assign(std::max(modspace, 2) + 2, 0.0);
+    ycoef.assign(2 * (nwant + modspace) + 4, 0.0);
+    epsmsr = std::sqrt(kEps);
+    *neig = 0;
+    *iter = 0;
+    iterp = 0;
+    istep = 0;
+    eta1 = kInf;
+    eta0 = sigma;
+    delta1 = kInf;
+    if (iserch == 0) eta0 = kInf;
+    nreig = kInf;
+    resid = kInf;
+    rndind = 0.0;
+    midspace = (modspace + 1) / 2;
+    if (ninit > midspace) {
+        initflg = false;
+        for (int q = 1; q < ninit; ++q)
+            if (eigs[q] != eigs[0]) initflg = true;
+        if (initflg) {
+            sortev_dev(n, ninit, xx, eigs, res, iserch, sigma);
+        } else {
+            ctx_->ensure_tmp(n);
+            for (int q = 1; q <= ninit / 2; ++q) {
+                int r = ninit - q + 1;
+                JDHIP_CHECK(hipMemcpyAsync(ctx_->d_tmp, D((long)(r - 1) * n), nb,
+                                           hipMemcpyDeviceToDevice, s));
+                JDHIP_CHECK(hipMemcpyAsync(D((long)(r - 1) * n), D((long)(q - 1) * n), nb,
+                                           hipMemcpyDeviceToDevice, s));
+                JDHIP_CHECK(hipMemcpyAsync(D((long)(q - 1) * n), ctx_->d_tmp, nb,
+                                           hipMemcpyDeviceToDevice, s));
+            }
+        }
+    }
+    initflg = false;
+
+L2:
+    if (ninit <= *neig) {
+        J = 1;
+        fill(s, D(vv), 1.0, n);
+    } else {
+        J = std::min(ninit - *neig, midspace);
+        IJ = ninit - *neig - J;
+        if (IJ > 0) initflg = true;
+        JDHIP_CHECK(hipMemcpyAsync(D(vv), D((long)n * IJ), nb * J, hipMemcpyDeviceToDevice, s));
+    }
+    IJ = 0;
+L4:
+    ++IJ;
+    repgs(n, kxp, *neig + IJ - 1, vv + (long)(IJ - 1) * n, ycoef, 1);
+    if (*neig + IJ - 1 > 0) sclr1 = ycoef[*neig + IJ - 1];
+    else sclr1 = dnorm(D(vv), n);
+    scal(s, D(vv + (long)(IJ - 1) * n), 1.0 / sclr1, n);
+    *ndx1 = vv + (long)(IJ - 1) * n;
+    *ndx2 = ww + (long)(IJ - 1) * n;
+    rlbl = 1;
+    if (internal_mv()) {
+        do_matvec(D(*ndx1), D(*ndx2));
+        goto L5;
+    }
+    *ijob = 1;
+    return;
+L5:
+    ++*iter;
+    if (IJ < J) goto L4;
+    axpyalpha = -sigma;
+    gram_initial(n);
+    uknown = (J <= 1);
+
+L10:  // extraction
+    MP = MM;
+    if (iserch == 2) NP = NN;
+    *info = solprojep(J, MP.data(), NP.data(), &theta, &theta2, REIGW, iserch);
+    if (*info != 0) goto L192;
+    if (!uknown) {
+        if (iserch == 2) {
+            sclr1 = small_nrm2(J, YY());
+            for (int c = 0; c < J; ++c) YY()[c] *= 1.0 / sclr1;
+        }
+        put(ctx_->slot(3), YY(), J);
+        gemv_n(s, rb, D(uu), D(vv), n, J, ctx_->slot(3), 1.0, 0.0, n, nullptr);
+        gemv_n(s, rb, D(reig), D(ww), n, J, ctx_->slot(3), 1.0, 0.0, n, nullptr);
+    } else {
+        YY()[0] = 1.0;
+        for (int c = 1; c < J; ++c) YY()[c] = 0.0;
+        JDHIP_CHECK(hipMemcpyAsync(D(uu), D(vv), nb, hipMemcpyDeviceToDevice, s));
+        JDHIP_CHECK(hipMemcpyAsync(D(reig), D(ww), nb, hipMemcpyDeviceToDevice, s));
+    }
+    if (iserch == 2) {
+        symv_upper(J, MM.data(), modspace, YY(), WL.data());
+        thmt = small_dot(J, WL.data(), YY());
+        theta = thmt + sigma;
+    } else {
+        thmt = theta;
+        eta0 = std::min(eta0, theta);
+    }
+    axpyalpha = -thmt;
+    {
+        double *d = ctx_->slot(2);
+        axpy_nrm2(s, rb, D(reig), axpyalpha, D(bu), n, d);
+        double v;
+        fetch(&v, d, 1);
+        nreigp = nreig;
+        nreig = std::sqrt(v);
+    }
+    if (neweig) {
+        rinit = nreig;
+        neweig = false;
+    }
+    if (std::fabs(nreig / nreigp - 1.0) < 0.01) ++istag;
+    else istag = 0;
+    if (nreig >= tol && istag <= 2) goto L20;
+    if (nreig >= tol && rndind <= 1.0) goto L20;
+    if (nreig >= tol && resid > 0.1 * nreig) goto L20;
+    // convergence detected
+    repgs(n, kxp, *neig, uu, ycoef, 0);
+    if (*neig > 0 && ycoef[*neig] == 0.0) goto L195;
+    sclr1 = (*neig > 0) ? ycoef[*neig] : dnorm(D(uu), n);
+    scal(s, D(uu), 1.0 / sclr1, n);
+
+L18:  // lock the converged pair
+    ++*neig;
+    kxp = uu;
+    kbxp = bu;
+    uu -= n;
+    bu -= n;
+    eigs[nwant - *neig] = theta;
+    res[nwant - *neig] = nreig;
+    rinitm *= rinit;
+    iterp = *iter;
+    *gap = theta2 - theta;
+    if (*neig >= nwant) goto L200;
+    delta1 = kInf;
+    nreig = kInf;
+    resid = kInf;
+    rndind = 0.0;
+    if (iserch != 2) {
+        eta1 = std::min(eta0, eta1);
+        // JDaux.F:1794-1796 reads EIGS(1..NEIG) (caller's input values for
+        // not-yet-filled slots) although converged values sit at the top.
+        for (int q = 0; q < *neig; ++q) eta0 = std::max(eta0, eigs[q]);
+    }
+    if (J == 1 || initflg) goto L2;
+    K = J;
+    J = std::min(midspace + 1, J - 1);
+    J = std::min(J, modspace - 1);
+    if (iserch == 2) {
+        // orthogonalise MP columns 2..J+1 against YY (small, host)
+        for (IJ = 2; IJ <= J + 1; ++IJ) {
+            double *w = MP.data() + (long)modspace * (IJ - 1);
+            const double *v = YY();
+            bool nrnd = true;
+            double nrn = small_nrm2(K, w), nro, nr = kEps * nrn;
+            for (;;) {
+                nro = nrn;
+                double c = small_dot(K, v, w);
+                for (int t = 0; t < K; ++t) w[t] -= c * v[t];
+                nrn = small_nrm2(K, w);
+                if (nrn <= nr) break;
+                if (!(nrn < 0.5 * nro)) break;
+            }
+            (void)nrnd;
+        }
+    }
+    *info = shrink(n, J, K, modspace /* YY2 */, iserch);
+    if (*info != 0) goto L193;
+    uknown = true;
+    neweig = true;
+    goto L10;
+
+L20:  // expansion
+    trs = std::sqrt(1.0 / 10);
+    if (istag > 0) trs = 1.0 / 10;
+    ++istep;
+    if (*iter >= mvmax - *neig - 1) goto L197;
+    if (J >= modspace) {
+        K = J;
+        J = midspace;
+        *info = shrink(n, J, K, 0 /* MP */, iserch);
+        if (*info != 0) goto L193;
+        double am[2], vjj;
+        argmax_abs(s, rb, D(uu), n, ctx_->slot(2));
+        fetch(am, ctx_->slot(2), 2);
+        JJ = (int)am[0];
+        fetch(&vjj, D(vv + JJ), 1);
+        axpyalpha = am[1] / vjj;
+        YY()[0] = axpyalpha / std::fabs(axpyalpha);
+        for (int c = 1; c < J; ++c) YY()[c] = 0.0;
+    }
+    eta = eta0;
+    if (J > 1) {
+        deltao = delta1;
+        if (iserch == 2) {
+            sclr1 = small_nrm2(J, YY2());
+            for (int c = 0; c < J; ++c) YY2()[c] *= 1.0 / sclr1;
+            symv_upper(J, MM.data(), modspace, YY2(), WL.data());
+            theta2 = small_dot(J, YY2(), WL.data());
+            delta1 = std::fabs(theta2) - std::fabs(theta - sigma);
+        } else {
+            delta1 = theta2 - theta;
+        }
+        if (nreig < std::fabs(delta1) && std::fabs(delta1 / deltao - 1.0) < 0.1) eta = theta;
+    }
+    repgs(n, bu, *neig + 1, reig, ycoef, -1);
+    rndind = std::fabs(ycoef[*neig] / (nreig * epsmsr));
+    resid = nreig;
+    xcg = vv + (long)J * n;
+    ftflg = true;
+    itcg = mvmax - *iter - *neig - 1;
+    ifo = (iserch != 2) ? 1 : 0;
+    gk = resid;
+    rcurr = nreig;
+    if (*neig <= 0) {
+        shift1 = theta;
+        *gap = (J > 1) ? theta2 - theta : 0.0;
+    }
+
+    // ---- LSOLREVCOM init (JDaux.F:3086-3116) ----
+    {
+        ls_posdef = (ifo == 1);
+        ifo = 0;
+        ls_maxit = itcg;
+        ls_iter = 0;
+        if (ls_posdef) fill(s, D(xcg), 0.0, n);
+        InnerScalars &h = *ctx_->h_sc;
+        h.posdef = ls_posdef ? 1 : 0;
+        h.flag = 0;
+        h.ggk2 = gk * gk;
+        h.eta = eta;
+        JDHIP_CHECK(hipMemcpyAsync(ctx_->d_sc, &h, sizeof(InnerScalars), hipMemcpyHostToDevice,
+                                   s));
+        ctx_->sync();
+    }
+L10i:  // new inner iteration: preconditioner solve Z = P^-1 R
+    ++ls_iter;
+    ++inner_iters;
+    *ndx1 = wcg + n;  // Z
+    *ndx2 = wcg;      // R
+    rlbl = 4;
+    if (internal_pc()) {
+        ops_->precond(*this, D(*ndx1), D(*ndx2));
+        goto L38;
+    }
+    *ijob = 2;
+    return;
+L38:  // project Z against [u, X] ; rho = r.z ; direction
+    flagcg = true;
+    cg_project_rho(s, rb, D(wcg + n), D(uu), *neig + 1, D(wcg), n, ctx_->d_sc, ctx_->slot(0));
+    cg_direction(s, D(wcg + n), D(wcg + 2L * n), n, ctx_->d_sc, ls_iter);
+    *ndx1 = wcg + 2L * n;  // P
+    *ndx2 = wcg + 3L * n;  // Q
+    rlbl = 3;
+    if (internal_mv()) {
+        ops_->matvec(D(*ndx1), D(*ndx2), nullptr, eta, D(bu), &ctx_->d_sc->alprho);
+        goto L37b;
+    }
+    // external matvec: the breakdown test must be known before returning
+    JDHIP_CHECK(hipMemcpyAsync(ctx_->h_sc, ctx_->d_sc, sizeof(InnerScalars),
+                               hipMemcpyDeviceToHost, s));
+    ctx_->sync();
+    ++nsync;
+    if (ctx_->h_sc->flag == 1) goto L40i;
+    *ijob = 1;
+    return;
+L37:  // after an external matvec: Q -= eta P ; alprho = u.Q
+    axpy(s, D(wcg + 3L * n), -eta, D(wcg + 2L * n), n);
+    multidot(s, rb, D(bu), n, 1, D(wcg + 3L * n), n, &ctx_->d_sc->alprho);
+    ++*iter;
+    goto L4i;
+L37b:
+L4i:
+    cg_project_alpha(s, rb, D(wcg + 3L * n), D(bu), D(wcg + 2L * n), n, ctx_->d_sc);
+    if (ls_posdef) {
+        cg_update_posdef(s, rb, D(wcg), D(wcg + 3L * n), D(xcg), D(wcg + 2L * n), n,
+                         ctx_->d_sc);
+    } else {
+        qmr_update_r(s, rb, D(wcg), D(wcg + 3L * n), n, ctx_->d_sc);
+        qmr_update_x(s, rb, D(xcg), D(xcg2), D(wcg + 2L * n), n, ctx_->d_sc, ls_iter);
+    }
+    JDHIP_CHECK(hipMemcpyAsync(ctx_->h_sc, ctx_->d_sc, sizeof(InnerScalars),
+                               hipMemcpyDeviceToHost, s));
+    ctx_->sync();
+    ++nsync;
+    {
+        const InnerScalars &h = *ctx_->h_sc;
+        if (internal_mv() && h.flag != 1) ++*iter;
+        if (h.flag == 1) goto L40i;
+        if (h.flag == 2) {
+            alpha = h.pq;
+            goto L40i;
+        }
+        alpha = h.alpha;
+        gk = std::sqrt(h.gk2);
+        alprho = h.alprho;
+        // L33: stopping test
+        if (ftflg) ifo = -1;
+        if (ftflg && iserch == 2) ifo = -2;
+        axpyalpha = axpyalpha * alpha;
+        alprho = alprho * alpha;
+        stoptest(n, tol, &resid, alprho, gk, &theta, eta, &ifo, trs, h.xn2);
+        ftflg = false;
+        // L5 of LSOLREVCOM
+        if (ifo == 1) {
+            ifo = 0;
+            goto L34;
+        }
+        if (ls_iter == ls_maxit) goto L40i;
+        goto L10i;
+    }
+L40i:  // inner iteration fails (JDaux.F:3255-3266)
+    ifo = 1;
+    if (alpha <= 0.0) ifo = 2;
+    if (ls_iter == 1)
+        JDHIP_CHECK(hipMemcpyAsync(D(xcg), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
+L34:
+    itcg = ls_iter;
+    if (iserch != 2 && eta <= eta0 && *neig > 0 && ifo == 2) eta0 = std::min(eta1, eta0);
+    conv = false;
+    flagcg = false;
+    rcum = rcum + std::log(nreig / resid);
+    if (resid >= tol) goto L50;
+    // convergence reached during the inner solve
+    axpy(s, D(uu), -1.0, D(xcg), n);
+    repgs(n, kxp, *neig, uu, ycoef, 0);
+    if (*neig > 0 && ycoef[*neig] == 0.0) goto L195;
+    sclr1 = (*neig > 0) ? ycoef[*neig] : dnorm(D(uu), n);
+    scal(s, D(uu), 1.0 / sclr1, n);
+    theta = theta / sclr1 + eta;
+    conv = true;
+    nreig = resid;
+    if (*neig >= nwant - 1) goto L18;
+L50:  // expand the subspace with the correction
+    repgs(n, kxp, *neig + J, xcg, ycoef, 1);
+    for (int c = 0; c < J; ++c) YY()[c] += -1.0 * ycoef[*neig + c];
+    sclr1 = ycoef[*neig + J];
+    scal(s, D(xcg), 1.0 / sclr1, n);
+    *ndx1 = xcg;
+    *ndx2 = ww + (long)J * n;
+    rlbl = 6;
+    if (internal_mv()) {
+        do_matvec(D(*ndx1), D(*ndx2));
+        goto L55;
+    }
+    *ijob = 1;
+    return;
+L55:
+    ++*iter;
+    YY()[J] = -sclr1;
+    axpyalpha = -sigma;
+    if (iserch == 2) axpy(s, D(*ndx2), axpyalpha, D(bv + (long)J * n), n);
+    {
+        double buf[2 * kMaxCols];
+        multidot(s, rb, D(vv), n, J + 1, D(*ndx2), n, ctx_->slot(0));
+        if (iserch == 2) multidot(s, rb, D(ww), n, J + 1, D(*ndx2), n, ctx_->slot(1));
+        fetch(buf, ctx_->slot(0), iserch == 2 ? 2 * kMaxCols : J + 1);
+        for (int r = 0; r <= J; ++r) MM[r + (long)J * modspace] = buf[r];
+        if (iserch == 2)
+            for (int r = 0; r <= J; ++r) NN[r + (long)J * modspace] = buf[kMaxCols + r];
+    }
+    ++J;
+    uknown = false;
+    if (!conv) goto L10;
+    // convergence detected: prepare for restart (JDaux.F:2040-2081)
+    sclr1 = small_nrm2(J, YY());
+    for (int c = 0; c < J; ++c) YY()[c] *= 1.0 / sclr1;
+    if (iserch == 2) {
+        symv_upper(J, NN.data(), modspace, YY(), WL2.data());
+        sclr1 = small_dot(J, YY(), WL2.data());
+        symv_upper(J, MM.data(), modspace, YY(), WL.data());
+        thmt = small_dot(J, YY(), WL.data());
+        axpyalpha = -thmt / (2 * sclr1);
+        for (int c = 0; c < J; ++c) WL[c] += axpyalpha * WL2[c];
+        axpyalpha = -1.0 / sclr1;
+        MP = MM;
+        syr2_upper(J, axpyalpha, WL.data(), WL2.data(), MP.data(), modspace);
+        theta = thmt + sigma;
+        NP = NN;
+    } else {
+        symv_upper(J, MM.data(), modspace, YY(), WL.data());
+        theta = small_dot(J, YY(), WL.data());
+        eta0 = std::min(eta0, theta);
+        sclr1 = small_nrm2(J, MP.data(), modspace + 1);
+        if (theta < 0) sclr1 = sclr1 - 2 * theta;
+        axpyalpha = sclr1;
+        for (int c = 0; c < J; ++c) WL[c] += axpyalpha * YY()[c];
+        for (int c = 0; c < J; ++c) WL2[c] = YY()[c];
+        MP = MM;
+        syr2_upper(J, -1.0, WL2.data(), WL.data(), MP.data(), modspace);
+    }
+    *info = solprojep(J, MP.data(), NP.data(), &sclr1, &sclr2, REIGW, -iserch);
+    if (*info != 0) goto L192;
+    if (iserch == 2) {
+        sclr1 = small_nrm2(J, YY());
+        for (int c = 0; c < J; ++c) YY()[c] *= 1.0 / sclr1;
+    }
+    goto L18;
+
+    // ---- error exits (JDaux.F:2090-2160) ----
+L190:
+    goto L301;
+L192:
+    if (*info > 0) *info = -*info;
+    goto L198;
+L193:
+    if (*info > 0) *info = -*info;
+    goto L198;
+L195:
+    *info = -54320;
+    goto L198;
+L197:
+    *info = *neig + 1;
+    rinitm *= rinit;
+L198:
+    if (*neig > 0) JDHIP_CHECK(hipMemcpyAsync(D(vv), D(uu), nb, hipMemcpyDeviceToDevice, s));
+L200:  // normal termination
+    if (*neig < nwant) {
+        rinit = std::pow(rinitm, 1.0 / (double)(*neig + 1));
+        if (*neig > 0) {
+            JJ = std::min(*neig, nwant - *neig);
+            JDHIP_CHECK(hipMemcpyAsync(D(xx), D((long)n * (nwant - JJ) + xx), nb * JJ,
+                                       hipMemcpyDeviceToDevice, s));
+            std::memmove(eigs, eigs + nwant - JJ, sizeof(double) * JJ);
+            std::memmove(res, res + nwant - JJ, sizeof(double) * JJ);
+            JDHIP_CHECK(hipMemcpyAsync(D((long)n * (*neig) + xx), D(vv), nb,
+                                       hipMemcpyDeviceToDevice, s));
+            eigs[*neig] = theta;
+            res[*neig] = nreig;
+            theta2 = theta;
+        } else {
+            JDHIP_CHECK(hipMemcpyAsync(D(xx), D(uu), nb, hipMemcpyDeviceToDevice, s));
+            eigs[0] = theta;
+            res[0] = nreig;
+            *gap = 0.0;
+            goto L300;
+        }
+    } else {
+        rinit = std::pow(rinitm, 1.0 / (double)(*neig));
+    }
+    sortev_dev(n, *neig, xx, eigs, res, 4, 0.0);
+    if (iserch >= 2) {
+        for (itarg = 1; itarg <= *neig; ++itarg)
+            if (eigs[itarg - 1] > sigma) break;
+    } else {
+        itarg = 0;
+    }
+    if (iserch == 2)
+        *gap = std::fabs(theta2 - sigma) -
+               std::max(std::fabs(eigs[*neig - 1] - sigma), std::fabs(eigs[0] - sigma));
+    else
+        *gap = theta2 - eigs[*neig - 1];
+    gg = (iserch == 2) ? eigs[0] + eigs[*neig - 1] - 2 * sigma : kInf;
+    if (gg > 0.0) {
+        II = 1;
+        JJ = *neig - 1;
+    } else {
+        II = 2;
+        JJ = *neig;
+    }
+    if (*neig == 1) {
+        II = 0;
+        JJ = 0;
+    }
+    I = 1;
+    K = 0;
+L210:
+    if (I < II || I >= JJ) goto L250;
+    gl = (I == 1) ? gg - res[*neig - 1] : eigs[I - 1] - eigs[I - 2] - res[I - 2];
+    if (gl < 10 * (eigs[I] - eigs[I - 1])) goto L250;
+    J = std::min(I + modspace, JJ + 1);
+L215:
+    --J;
+    if (J == I) goto L250;
+    gr = (J == *neig) ? -gg - res[0] : eigs[J] - eigs[J - 1] - res[J];
+    if (10 * (eigs[J - 1] - eigs[I - 1]) > std::min(gl, gr)) goto L215;
+    // cluster detected: additional Rayleigh-Ritz
+    ++K;
+    J = J - I + 1;
+    JDHIP_CHECK(hipMemcpyAsync(D(vv), D((long)n * (I - 1) + xx), nb * J,
+                               hipMemcpyDeviceToDevice, s));
+    IJ = 0;
+L220:
+    ++IJ;
+    *ndx1 = vv + (long)(IJ - 1) * n;
+    *ndx2 = ww + (long)(IJ - 1) * n;
+    rlbl = 2;
+    if (internal_mv()) {
+        do_matvec(D(*ndx1), D(*ndx2));
+        goto L225;
+    }
+    *ijob = 3;
+    return;
+L225:
+    ++*iter;
+    if (IJ < J) goto L220;
+    {
+        const int ld = modspace;
+        for (int c = 0; c < J; ++c) {
+            double buf[2 * kMaxCols];
+            multidot(s, rb, D(vv), n, c + 1, D(ww + (long)c * n), n, ctx_->slot(0));
+            multidot(s, rb, D(ww), n, c + 1, D(vv + (long)c * n), n, ctx_->slot(1));
+            fetch(buf, ctx_->slot(0), 2 * kMaxCols);
+            for (int r = 0; r <= c; ++r) MM[r + c * ld] = buf[r] + buf[kMaxCols + r];
+        }
+        for (auto &v : MM) v *= 0.5;
+    }
+    IJ = solprojep(J, MM.data(), NP.data(), &sclr1, &sclr2, REIGW, 3);
+    if (IJ != 0) goto L250;
+    {
+        std::vector<double> q((size_t)J * J);
+        for (int c = 0; c < J; ++c)
+            for (int r = 0; r < J; ++r) q[r + (size_t)c * J] = MM[r + (size_t)c * modspace];
+        double *dq = ctx_->d_small + 4096;
+        put(dq, q.data(), (long)J * J);
+        ctx_->ensure_tmp((long)n * J);
+        tall_gemm(s, D(vv), n, J, dq, J, J, ctx_->d_tmp, n, n);
+        JDHIP_CHECK(hipMemcpyAsync(D((long)n * (I - 1) + xx), ctx_->d_tmp, nb * J,
+                                   hipMemcpyDeviceToDevice, s));
+        tall_gemm(s, D(ww), n, J, dq, J, J, ctx_->d_tmp, n, n);
+        JDHIP_CHECK(hipMemcpyAsync(D(vv), ctx_->d_tmp, nb * J, hipMemcpyDeviceToDevice, s));
+    }
+    nrc = 0.0;
+    gl = gl - eigs[I - 1];
+    gr = gr + eigs[I + J - 2];
+    for (IJ = 1; IJ <= J; ++IJ) {
+        theta = NP[IJ - 1];
+        eigs[I + IJ - 2] = theta;
+        double v;
+        axpy_nrm2(s, rb, D(vv + (long)n * (IJ - 1)), -theta, D((long)n * (I + IJ - 2) + bx), n,
+                  ctx_->slot(2));
+        fetch(&v, ctx_->slot(2), 1);
+        nreig = std::sqrt(v);
+        res[I + IJ - 2] = nreig;
+        nrc += nreig * nreig;
+    }
+    for (IJ = I; IJ <= I + J - 1; ++IJ)
+        if (IJ == itarg) --itarg;
+    I = I + J;
+    if (I > *neig) goto L300;
+    goto L210;
+L250:  // isolated eigenvalue: true residual
+    *ndx1 = (long)n * (I - 1);
+    *ndx2 = vv;
+    rlbl = 5;
+    if (internal_mv()) {
+        do_matvec(D(*ndx1), D(*ndx2));
+        goto L255;
+    }
+    *ijob = 3;
+    return;
+L255:
+    ++*iter;
+    {
+        double d[1], v;
+        dot(s, rb, D((long)n * (I - 1)), D(vv), n, ctx_->slot(2));
+        fetch(d, ctx_->slot(2), 1);
+        theta = d[0];
+        axpy_nrm2(s, rb, D(vv), -theta, D((long)n * (I - 1) + bx), n, ctx_->slot(2));
+        fetch(&v, ctx_->slot(2), 1);
+        nreig = std::sqrt(v);
+    }
+    eigs[I - 1] = theta;
+    res[I - 1] = nreig;
+    if (I == itarg) --itarg;
+    ++I;
+    if (I <= *neig) goto L210;
+L300:
+L301:
+    *ijob = 0;
+    ctx_->sync();
+}
+
+}  // namespace jdhip

@@ -1,0 +1,143 @@
+// engine.hpp -- device-resident Jacobi-Davidson engine (host control).
+//
+// JdEngine restates JDRVCOM (src/jd/JDaux.F:1391-2415) with its inner solver
+// LSOLREVCOM (2989-3268) and STOPTEST (2542-2672) as an explicit-state
+// coroutine.  All n-length data lives in a device workspace X laid out like
+// the reference's X array (so reverse-communication offsets NDX1/NDX2 mean
+// the same thing); the J x J projected matrices, the Ritz coefficients and
+// the scalar control stay on the host.  Matrix-vector products and
+// preconditioner solves are either returned to the caller (reverse
+// communication) or executed inline through an Ops object, in which case the
+// inner iteration runs as a short chain of fused kernels with one host
+// synchronisation per step.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace jdhip {
+
+#define JDHIP_CHECK(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(_e) + \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+// Per-device resources of a solver context.
+struct DevCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    RedBuf rb{};
+    double *d_small = nullptr;   // 4 x kMaxCols coefficient slots + scalars
+    InnerScalars *d_sc = nullptr;
+    double *h_pin = nullptr;     // pinned host staging (4 x kMaxCols + 8)
+    InnerScalars *h_sc = nullptr;
+    double *d_tmp = nullptr;     // n x ncols scratch (tall products)
+    long tmp_len = 0;
+
+    void init(int dev);
+    void release();
+    void ensure_tmp(long len);
+    void sync() { JDHIP_CHECK(hipStreamSynchronize(stream)); }
+    // device slots
+    double *slot(int k) { return d_small + (long)k * kMaxCols; }
+};
+
+class JdEngine;
+
+// Inline operators.  When present, the engine executes them without
+// returning to the caller.
+struct Ops {
+    virtual ~Ops() = default;
+    virtual bool has_matvec() const { return false; }
+    // y = A x ; if eta_dev/eta: y -= eta x ; if u: *udot_dev = u.y
+    virtual void matvec(const double *x, double *y, const double *eta_dev, double eta,
+                        const double *u, double *udot_dev) {}
+    virtual bool has_precond() const { return false; }
+    // called when the reference would return IJOB=2 to PJDRVCOM
+    virtual void precond(JdEngine &e, double *z, const double *r) {}
+};
+
+class JdEngine {
+  public:
+    explicit JdEngine(DevCtx *ctx) : ctx_(ctx) {}
+
+    // One JDRVCOM call.  x is the DEVICE workspace (lx doubles), eigs/res
+    // host arrays.  On return *ijob: 0 done, 1 matvec X(ndx2)=A X(ndx1),
+    // 2 precond X(ndx1)=P^-1 X(ndx2), 3 matvec (final checks).  ndx are
+    // 0-based offsets into x.
+    void step(int n, double *eigs, double *res, double *x, long lx, int *neig, double sigma,
+              int isearch, int ninit, int madspace, int *iter, double tol, int *ijob,
+              long *ndx1, long *ndx2, int iprint, int *info, double *gap);
+
+    void set_ops(Ops *ops) { ops_ = ops; }
+
+    // COMMON /JDRVCOMJD/ (JDaux.F:46-47): read by the preconditioner manager
+    double timeilu = 0, rinit = 0, rcurr = 0, rcum = 0, shift1 = 0;
+    bool flagcg = false;
+    long inner_iters = 0;  // total inner steps (diagnostic)
+    long nsync = 0;        // host synchronisations (diagnostic)
+
+  private:
+    // ---- device helpers ----
+    double dnorm(const double *w, long n);
+    void repgs(int n, long voff, int jv, long woff, std::vector<double> &y, int gam);
+    void sortev_dev(int n, int j, long voff, double *eigs, double *r, int it, double sigma);
+    int solprojep(int j, double *mm, double *nn, double *theta, double *theta2,
+                  std::vector<double> &w, int is);
+    int shrink(int n, int j, int k, int ucol_off, int iserch);
+    void gram_initial(int n);
+    void fetch(double *host, const double *dev, long cnt);
+    void put(double *dev, const double *host, long cnt);
+    void stoptest(int n, double tol, double *rk, double alprho, double gk, double *theta,
+                  double eta, int *info, double trs, double xn2);
+    bool internal_mv() const { return ops_ && ops_->has_matvec(); }
+    bool internal_pc() const { return ops_ && ops_->has_precond(); }
+    void do_matvec(const double *xin, double *yout) {
+        ops_->matvec(xin, yout, nullptr, 0.0, nullptr, nullptr);
+    }
+
+    DevCtx *ctx_;
+    Ops *ops_ = nullptr;
+    double *X_ = nullptr;  // device workspace of the current call
+
+    // host small matrices: MM, NN, MP, NP (modspace^2 each) and work
+    std::vector<double> MM, NN, MP, NP, WL, WL2, WL3, REIGW;
+    double *YY() { return MP.data(); }
+    double *YY2() { return MP.data() + modspace; }
+
+    // ---- JDRVCOM locals (SAVE) ----
+    int rlbl = 0;
+    bool ftflg = false, conv = false, uknown = false, initflg = false, neweig = false;
+    int mvmax = 0, midspace = 0, modspace = 0, nwant = 0, itarg = 0, iserch = 0;
+    long xx = 0, xcg = 0, xcg2 = 0, wcg = 0, vv = 0, ww = 0, uu = 0, reig = 0;
+    long bx = 0, bu = 0, bv = 0, kxp = 0, kbxp = 0;
+    int iterp = 0, itcg = 0, ifo = 0, istep = 0, istag = 0;
+    int I = 0, IJ = 0, J = 0, K = 0, II = 0, JJ = 0;
+    double resid = 0, sclr1 = 0, sclr2 = 0, theta = 0, eta = 0, thmt = 0, delta1 = 0,
+           deltao = 0, eta0 = 0, nreig = 0, theta2 = 0, eta1 = 0, epsmsr = 0, nreigp = 0,
+           trs = 0, rndind = 0, gr = 0, gl = 0, gg = 0, nrc = 0, rinitm = 0, alpha = 0,
+           gk = 0, alprho = 0, axpyalpha = 0;
+    std::vector<double> ycoef;
+
+    // ---- LSOLREVCOM locals ----
+    int ls_iter = 0, ls_maxit = 0;
+    bool ls_posdef = false;
+
+    // ---- STOPTEST locals (SAVE) ----
+    int st_istag = 0, st_istt = 0;
+    bool st_toupdate = false;
+    double st_g0 = 0, st_gs = 0, st_gtrs = 0, st_gkmin = 0, st_bk = 0, st_beta = 0,
+           st_tn = 0, st_un2 = 0, st_gkm1 = 0, st_alp = 0;
+
+    unsigned long long rng_ = 0x2545F4914F6CDD1DULL;
+};
+
+}  // namespace jdhip

@@ -1,0 +1,286 @@
+"""Python mirrors of the reference drivers over the native C-ABI.
+
+Same argument names and meaning as src/jd/PJD.F (DPJD, DPJDREVCOM, DJDREVCOM,
+DPJDCLEANUP); reverse communication is serviced by a Python callable for the
+legacy host-array entry points, or on the device (jdhip_spmv) for the
+context API.  All numerics run in libjadamilu_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import JdhipStats, lib
+
+_i = C.c_int
+_d = C.c_double
+
+
+def lx_optimal(n, neig, madspace):
+    """Optimal workspace length N*(3*MADSPACE+NEIG+1)+3*MADSPACE**2+MAX(MADSPACE**2,NEIG)
+    (PJD.F:99-103)."""
+    return n * (3 * madspace + neig + 1) + 3 * madspace ** 2 + max(madspace ** 2, neig)
+
+
+@dataclass
+class JDResult:
+    eigs: np.ndarray
+    res: np.ndarray
+    x: object
+    neig: int
+    iter: int
+    info: int
+    gap: float
+    shift: float = 0.0
+    droptol: float = 0.0
+    stats: dict = field(default_factory=dict)
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _icntl(icntl):
+    return np.array(list(icntl) + [0] * (5 - len(icntl)), dtype=np.int32)
+
+
+def pjdrevcom_host(n, a, ja, ia, matvec, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20,
+                   maxiter=1000, tol=1e-10, shift=0.0, droptol=1e-3, mem=20.0,
+                   icntl=(0, 0, 0, 0, 0), x0=None, eigs0=None, lx=None, iprint=0):
+    """DPJDREVCOM through the legacy symbol `dpjdrevcom_` (host X).
+
+    `matvec(xin, xout)` must write A@xin into xout.  `ja[0] < 0` selects the
+    diagonal preconditioner with `a` the diagonal (PJD.F:309-317)."""
+    L = lib()
+    lx = lx or lx_optimal(n, neig, madspace)
+    x = np.zeros(lx)
+    if x0 is not None:
+        x[: np.size(x0)] = np.asarray(x0).ravel(order="F")
+    a = np.array(a, dtype=np.float64)
+    ja = np.array(ja, dtype=np.int32)
+    ia = np.array(ia, dtype=np.int32)
+    eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+    res = np.zeros(max(neig, 1))
+    ic = _icntl(icntl)
+    N, LX, NE, IS, NI, MS, IT = _i(n), _i(lx), _i(neig), _i(isearch), _i(ninit), _i(madspace), _i(maxiter)
+    SG, TL, SH, DT, ME = _d(sigma), _d(tol), _d(shift), _d(droptol), _d(mem)
+    IJ, N1, N2, IP, INFO, GAP = _i(0), _i(0), _i(0), _i(iprint), _i(0), _d(0)
+    nmv = 0
+    while True:
+        L.dpjdrevcom_(C.byref(N), _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(eigs), _p(res),
+                      _p(x), C.byref(LX), C.byref(NE), C.byref(SG), C.byref(IS), C.byref(NI),
+                      C.byref(MS), C.byref(IT), C.byref(TL), C.byref(SH), C.byref(DT),
+                      C.byref(ME), _p(ic, C.c_int), C.byref(IJ), C.byref(N1), C.byref(N2),
+                      C.byref(IP), C.byref(INFO), C.byref(GAP))
+        if IJ.value != 1:
+            break
+        a1, a2 = N1.value - 1, N2.value - 1
+        matvec(x[a1:a1 + n], x[a2:a2 + n])
+        nmv += 1
+    k = NE.value if INFO.value >= 0 else 0
+    return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(),
+                    x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=IT.value,
+                    info=INFO.value, gap=GAP.value, shift=SH.value, droptol=DT.value,
+                    stats={"nmv": nmv})
+
+
+def jdrevcom_host(n, matvec, psolve, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20,
+                  maxiter=1000, tol=1e-10, x0=None, eigs0=None, lx=None, iprint=0):
+    """DJDREVCOM through the legacy symbol `djdrevcom_` (host X).
+
+    `psolve(src, dst)` must write P^-1 src into dst."""
+    L = lib()
+    lx = lx or lx_optimal(n, neig, madspace)
+    x = np.zeros(lx)
+    if x0 is not None:
+        x[: np.size(x0)] = np.asarray(x0).ravel(order="F")
+    eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+    res = np.zeros(max(neig, 1))
+    N, LX, NE, IS, NI, MS, IT = _i(n), _i(lx), _i(neig), _i(isearch), _i(ninit), _i(madspace), _i(maxiter)
+    SG, TL = _d(sigma), _d(tol)
+    IJ, N1, N2, IP, INFO, GAP = _i(0), _i(0), _i(0), _i(iprint), _i(0), _d(0)
+    while True:
+        L.djdrevcom_(C.byref(N), _p(eigs), _p(res), _p(x), C.byref(LX), C.byref(NE),
+                     C.byref(SG), C.byref(IS), C.byref(NI), C.byref(MS), C.byref(IT),
+                     C.byref(TL), C.byref(IJ), C.byref(N1), C.byref(N2), C.byref(IP),
+                     C.byref(INFO), C.byref(GAP))
+        a1, a2 = N1.value - 1, N2.value - 1
+        if IJ.value == 1:
+            matvec(x[a1:a1 + n], x[a2:a2 + n])
+        elif IJ.value == 2:
+            psolve(x[a2:a2 + n], x[a1:a1 + n])
+        else:
+            break
+    k = NE.value if INFO.value >= 0 else 0
+    return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(),
+                    x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=IT.value,
+                    info=INFO.value, gap=GAP.value)
+
+
+def pjd_host(n, a, ja, ia, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20, maxiter=1000,
+             tol=1e-10, shift=0.0, droptol=1e-3, mem=20.0, icntl=(0, 0, 0, 0, 0), x0=None,
+             eigs0=None, lx=None, iprint=0):
+    """DPJD through the legacy symbol `dpjd_` (host arrays, device solve)."""
+    L = lib()
+    lx = lx or lx_optimal(n, neig, madspace)
+    x = np.zeros(lx)
+    if x0 is not None:
+        x[: np.size(x0)] = np.asarray(x0).ravel(order="F")
+    a = np.array(a, dtype=np.float64)
+    ja = np.array(ja, dtype=np.int32)
+    ia = np.array(ia, dtype=np.int32)
+    eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+    res = np.zeros(max(neig, 1))
+    ic = _icntl(icntl)
+    N, LX, NE, IS, NI, MS, IT = _i(n), _i(lx), _i(neig), _i(isearch), _i(ninit), _i(madspace), _i(maxiter)
+    SG, TL, SH, DT, ME = _d(sigma), _d(tol), _d(shift), _d(droptol), _d(mem)
+    IP, INFO, GAP = _i(iprint), _i(0), _d(0)
+    L.dpjd_(C.byref(N), _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(eigs), _p(res), _p(x),
+            C.byref(LX), C.byref(NE), C.byref(SG), C.byref(IS), C.byref(NI), C.byref(MS),
+            C.byref(IT), C.byref(TL), C.byref(SH), C.byref(DT), C.byref(ME), _p(ic, C.c_int),
+            C.byref(IP), C.byref(INFO), C.byref(GAP))
+    k = NE.value if INFO.value >= 0 else 0
+    return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(),
+                    x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=IT.value,
+                    info=INFO.value, gap=GAP.value, shift=SH.value, droptol=DT.value)
+
+
+def cleanup():
+    lib().dpjdcleanup_()
+
+
+class Context:
+    """A solver context on one HIP device (jdhip_create); X lives in device
+    memory (a torch CUDA tensor is used as the allocator)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self.device = device
+        self.h = self._L.jdhip_create(device)
+        if not self.h:
+            raise RuntimeError("jdhip_create failed")
+
+    def close(self):
+        if self.h:
+            self._L.jdhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        return self._L.jdhip_last_error(self.h).decode()
+
+    def stats(self):
+        s = JdhipStats()
+        self._L.jdhip_get_stats(self.h, C.byref(s))
+        return {"inner_iters": s.inner_iters, "host_syncs": s.host_syncs,
+                "n_precond_setups": s.n_precond_setups, "spmv_lanes": s.spmv_lanes,
+                "setup_shifts": list(s.setup_shifts[: s.n_precond_setups])}
+
+    def stream_ptr(self):
+        return self._L.jdhip_stream(self.h)
+
+    def set_matrix(self, ia, ja, a, lanes_per_row=0):
+        ia = np.ascontiguousarray(ia, dtype=np.int32)
+        ja = np.ascontiguousarray(ja, dtype=np.int32)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._keep = (ia, ja, a)
+        rc = self._L.jdhip_set_matrix(self.h, ia.size - 1, _p(ia, C.c_int), _p(ja, C.c_int),
+                                      _p(a), lanes_per_row)
+        if rc != 0:
+            raise RuntimeError(self.error())
+
+    def spmv(self, x_ptr, y_ptr):
+        rc = self._L.jdhip_spmv(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr))
+        if rc != 0:
+            raise RuntimeError(self.error())
+
+    def pjdrevcom(self, n, a, ja, ia, xdev_ptr, lx, *, neig, matvec=None, isearch=0, sigma=0.0,
+                  ninit=0, madspace=20, maxiter=1000, tol=1e-10, shift=0.0, droptol=1e-3,
+                  mem=20.0, icntl=(0, 0, 0, 0, 0), eigs0=None, iprint=0):
+        """jdhip_pjdrevcom with X on the device.  IJOB=1 is serviced by
+        `matvec(xin_ptr, xout_ptr)` or, if None, by the context's SpMV."""
+        L = self._L
+        a = np.array(a, dtype=np.float64)
+        ja = np.array(ja, dtype=np.int32)
+        ia = np.array(ia, dtype=np.int32)
+        eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+        res = np.zeros(max(neig, 1))
+        ic = _icntl(icntl)
+        NE, IT, SH, DT, ME = _i(neig), _i(maxiter), _d(shift), _d(droptol), _d(mem)
+        IJ, N1, N2, INFO, GAP = _i(0), C.c_long(0), C.c_long(0), _i(0), _d(0)
+        nmv = 0
+        while True:
+            L.jdhip_pjdrevcom(self.h, n, _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(eigs),
+                              _p(res), C.c_void_p(xdev_ptr), lx, C.byref(NE), sigma, isearch,
+                              ninit, madspace, C.byref(IT), tol, C.byref(SH), C.byref(DT),
+                              C.byref(ME), _p(ic, C.c_int), C.byref(IJ), C.byref(N1),
+                              C.byref(N2), iprint, C.byref(INFO), C.byref(GAP))
+            if IJ.value != 1:
+                break
+            xi = xdev_ptr + 8 * (N1.value - 1)
+            xo = xdev_ptr + 8 * (N2.value - 1)
+            if matvec is None:
+                self.spmv(xi, xo)
+            else:
+                matvec(xi, xo)
+            nmv += 1
+        k = NE.value if INFO.value >= 0 else 0
+        st = self.stats()
+        st["nmv"] = nmv
+        return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(), x=None, neig=k,
+                        iter=IT.value, info=INFO.value, gap=GAP.value, shift=SH.value,
+                        droptol=DT.value, stats=st)
+
+    def pjd(self, n, a, ja, ia, xdev_ptr, lx, *, neig, isearch=0, sigma=0.0, ninit=0,
+            madspace=20, maxiter=1000, tol=1e-10, shift=0.0, droptol=1e-3, mem=20.0,
+            icntl=(0, 0, 0, 0, 0), eigs0=None, iprint=0):
+        """jdhip_pjd: internal device SpMV and preconditioner."""
+        L = self._L
+        a = np.array(a, dtype=np.float64)
+        ja = np.array(ja, dtype=np.int32)
+        ia = np.array(ia, dtype=np.int32)
+        eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+        res = np.zeros(max(neig, 1))
+        ic = _icntl(icntl)
+        NE, IT, SH, DT, ME = _i(neig), _i(maxiter), _d(shift), _d(droptol), _d(mem)
+        INFO, GAP = _i(0), _d(0)
+        L.jdhip_pjd(self.h, n, _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(eigs), _p(res),
+                    C.c_void_p(xdev_ptr), lx, C.byref(NE), sigma, isearch, ninit, madspace,
+                    C.byref(IT), tol, C.byref(SH), C.byref(DT), C.byref(ME), _p(ic, C.c_int),
+                    iprint, C.byref(INFO), C.byref(GAP))
+        k = NE.value if INFO.value >= 0 else 0
+        return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(), x=None, neig=k,
+                        iter=IT.value, info=INFO.value, gap=GAP.value, shift=SH.value,
+                        droptol=DT.value, stats=self.stats())
+
+    def jdrevcom(self, n, xdev_ptr, lx, *, neig, matvec, psolve, isearch=0, sigma=0.0, ninit=0,
+                 madspace=20, maxiter=1000, tol=1e-10, eigs0=None, iprint=0):
+        """jdhip_jdrevcom: matvec(xin_ptr, xout_ptr), psolve(src_ptr, dst_ptr)."""
+        L = self._L
+        eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+        res = np.zeros(max(neig, 1))
+        NE, IT = _i(neig), _i(maxiter)
+        IJ, N1, N2, INFO, GAP = _i(0), C.c_long(0), C.c_long(0), _i(0), _d(0)
+        while True:
+            L.jdhip_jdrevcom(self.h, n, _p(eigs), _p(res), C.c_void_p(xdev_ptr), lx,
+                             C.byref(NE), sigma, isearch, ninit, madspace, C.byref(IT), tol,
+                             C.byref(IJ), C.byref(N1), C.byref(N2), iprint, C.byref(INFO),
+                             C.byref(GAP))
+            p1 = xdev_ptr + 8 * (N1.value - 1)
+            p2 = xdev_ptr + 8 * (N2.value - 1)
+            if IJ.value == 1:
+                matvec(p1, p2)
+            elif IJ.value == 2:
+                psolve(p2, p1)
+            else:
+                break
+        k = NE.value if INFO.value >= 0 else 0
+        return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(), x=None, neig=k,
+                        iter=IT.value, info=INFO.value, gap=GAP.value, stats=self.stats())

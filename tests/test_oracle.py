@@ -76,8 +76,9 @@ def test_laplace2d_closed_form_small():
     shape = (30, 30)
     ia, ja, a = M.laplace(shape)
     lam, nxt = M.laplace_eigs(shape, 6)
+    x0 = np.random.default_rng(0).random(900)
     r = O.pjdrevcom_diag(900, np.full(900, 4.0), O.matvec_upper(ia, ja, a), neig=6,
-                         madspace=20, maxiter=20000, tol=1e-10)
+                         madspace=20, maxiter=20000, tol=1e-10, ninit=1, x0=x0)
     assert r.info == 0
     np.testing.assert_allclose(r.eigs, lam, rtol=1e-10)
 
