@@ -105,6 +105,47 @@ int jdhip_set_matrix(jdhip_ctx *ctx, int n, const int *ia, const int *ja, const 
 /* y = A x on ctx's stream, x and y DEVICE arrays of n doubles. */
 int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
 
+/* ---- options ------------------------------------------------------------- */
+/* Build options of ctx (no reference counterpart; defaults reproduce the
+ * reference's behaviour): "ildl_sweeps" (0 = exact level-scheduled
+ * triangular sweeps, k > 0 = k Jacobi sweeps per triangular solve),
+ * "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee), "ildl_dense_max"
+ * (largest final level factored densely), "spmv_lanes".  0 on success. */
+int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value);
+
+/* z = P^-1 r with the preconditioner currently held by ctx (diagonal or
+ * multilevel ILDL); DEVICE arrays.  The apply of SYMILUPACKSOL
+ * (src/ilupack/symilupacksol.c:60-128) / DGLPRECSOL (src/jd/dglprecsol.c). */
+int jdhip_precond_apply(jdhip_ctx *ctx, double *z_dev, const double *r_dev);
+
+/* ---- multilevel ILDL factor (host; no device needed) --------------------- */
+/* Factor A - shift*I (A: 1-based upper CSR) as the reference's
+ * SYMILUPACKFACTOR / DSYMAMGfactor (src/ilupack/symilupackfactor.c:66-220,
+ * symAMGsetup.c:121-1647) does: multilevel inverse-based ILDL^T with
+ * droptol, condest, optional |D| conversion (spd, symspd.c:37) and FP32
+ * arithmetic (single, singlesymilupackfactor.c:78-121).  *ierr: 0 ok, -3
+ * elbow exceeded, >0 zero pivot in the final dense level. */
+typedef struct jdhip_ildl jdhip_ildl;
+jdhip_ildl *jdhip_ildl_factor(int n, const int *ia, const int *ja, const double *a, double shift,
+                              double droptol, double condest, int spd, int single, int ordering,
+                              int dense_max, int *ierr);
+void jdhip_ildl_free(jdhip_ildl *f);
+/* number of sparse levels; *dense_m = order of the dense final level */
+int jdhip_ildl_nlevels(jdhip_ildl *f, int *dense_m);
+int jdhip_ildl_level_sizes(jdhip_ildl *f, int lev, int *n, int *nB, long *nnz_lb, long *nnz_lc);
+/* copy out level lev: perm[n] (position -> level input index), scal[n],
+ * strict lower L_B (CSR, nB rows), L_CB (CSR, n-nB rows), pivots d[nB] */
+int jdhip_ildl_level_data(jdhip_ildl *f, int lev, int *perm, double *scal, int *lb_ptr,
+                          int *lb_idx, double *lb_val, int *lc_ptr, int *lc_idx, double *lc_val,
+                          double *d);
+/* dense final level: explicit inverse (m x m, row-major) */
+int jdhip_ildl_dense(jdhip_ildl *f, double *minv);
+int jdhip_ildl_stats(jdhip_ildl *f, double *fill, int *nneg, double *condest, double *seconds);
+/* upload f as the preconditioner of ctx (sweeps as "ildl_sweeps") */
+int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps);
+/* dependency levels of the uploaded factor and algorithmic bytes per apply */
+int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels, long *apply_bytes);
+
 /* ---- diagnostics --------------------------------------------------------- */
 typedef struct {
     long inner_iters;     /* inner PCG/sQMR steps of the last call */
@@ -112,6 +153,10 @@ typedef struct {
     int n_precond_setups; /* preconditioner (re)factorisations */
     int spmv_lanes;       /* lanes per row chosen for the SpMV */
     double setup_shifts[64];
+    double precond_seconds; /* host factorisation + upload time of the last call */
+    double fill;            /* stored factor entries / nnz(upper A) (ILDL) */
+    int precond_levels;     /* sparse levels + dense level (ILDL) */
+    int precond_negpiv;     /* negative pivots before |D| conversion */
 } jdhip_stats;
 void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out);
 /* last error message of ctx ("" if none) */

@@ -31,7 +31,9 @@ _v = C.c_void_p
 class JdhipStats(C.Structure):
     _fields_ = [("inner_iters", C.c_long), ("host_syncs", C.c_long),
                 ("n_precond_setups", C.c_int), ("spmv_lanes", C.c_int),
-                ("setup_shifts", C.c_double * 64)]
+                ("setup_shifts", C.c_double * 64), ("precond_seconds", C.c_double),
+                ("fill", C.c_double), ("precond_levels", C.c_int),
+                ("precond_negpiv", C.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -59,6 +61,18 @@ SIGNATURES = {
     "jdhip_set_matrix": (C.c_int, [_v, C.c_int, _i, _i, _d, C.c_int]),
     "jdhip_spmv": (C.c_int, [_v, _v, _v]),
     "jdhip_get_stats": (None, [_v, C.POINTER(JdhipStats)]),
+    "jdhip_set_option": (C.c_int, [_v, C.c_char_p, C.c_double]),
+    "jdhip_precond_apply": (C.c_int, [_v, _v, _v]),
+    "jdhip_ildl_factor": (_v, [C.c_int, _i, _i, _d, C.c_double, C.c_double, C.c_double, C.c_int,
+                               C.c_int, C.c_int, C.c_int, _i]),
+    "jdhip_ildl_free": (None, [_v]),
+    "jdhip_ildl_nlevels": (C.c_int, [_v, _i]),
+    "jdhip_ildl_level_sizes": (C.c_int, [_v, C.c_int, _i, _i, _l, _l]),
+    "jdhip_ildl_level_data": (C.c_int, [_v, C.c_int, _i, _d, _i, _i, _d, _i, _i, _d, _d]),
+    "jdhip_ildl_dense": (C.c_int, [_v, _d]),
+    "jdhip_ildl_stats": (C.c_int, [_v, _d, _i, _d, _d]),
+    "jdhip_ildl_attach": (C.c_int, [_v, _v, C.c_int]),
+    "jdhip_ildl_device_info": (C.c_int, [_v, _i, _i, _l]),
     "jdhip_last_error": (C.c_char_p, [_v]),
 }
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,8 +47,12 @@ struct Solver {
 
     // COMMON /PJDINITPJD/ (zero-initialised like the Fortran COMMON storage)
     double timefact = 0, shift0 = 0, diagmin = 0, shiftmax = 0;
+    double memrequested = 0, memused = 0, droptol0 = 0, condest0 = 0;
     double slightlyless = 0.8, toldiv = 2.0;  // src/blaslike/common.f:14-15
     int factdgl = 0, factspd = 0, prvdr = 0;
+    // build options of the multilevel ILDL
+    int ildl_sweeps = 0, ildl_ordering = 1, ildl_dense_max = 1500;
+    IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
     double rprev = 0;
@@ -73,6 +78,7 @@ struct Solver {
     int *ja = nullptr, *ia = nullptr, *neig = nullptr, *icntl = nullptr;
     std::vector<double> shifts_log;
     int nsetups = 0;
+    double precond_seconds = 0.0;
 
     explicit Solver(int dev) {
         ctx.init(dev);
@@ -105,10 +111,28 @@ struct Solver {
     }
 
     void upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr);
+    double A_fp = 0.0;
+    static double fingerprint(int n_, const int *ia_, const int *ja_, const double *a_) {
+        double s = (double)n_ + 1e-3 * (double)ia_[n_];
+        for (int i = 0; i < n_; ++i)
+            for (int k = ia_[i] - 1; k < ia_[i + 1] - 1; ++k)
+                s += (double)ja_[k] * 1e-7 + a_[k] * (1.0 + 1e-9 * (double)(k % 977));
+        return s;
+    }
     int autotune_lpr();
 
     // PJDINIT (JDaux.F:549-1325)
     void pjdinit(int isearch_, double sigma_, double shift_in);
+    void pjdinit_ilu(int isearch_, double sigma_);
+    double savediag_shift() const;
+    void suggest(int NE, const double *ev, double gp);
+    double tol_ = 1e-10;
+    double *gap_ = nullptr;
+    double gap_dummy_ = 0.0;
+    std::chrono::steady_clock::time_point tic_ = std::chrono::steady_clock::now();
+    static double seconds_since(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    }
     void dglprecsetup(int isearch_, double shift_in, double *shift0_out, double *diagmin_out);
     void precond_request(double *z, const double *r);
 
@@ -261,32 +285,131 @@ void Solver::pjdinit(int isearch_, double sigma_, double shift_in) {
         ++nsetups;
         return;
     }
-    // multilevel ILDL branch
-    IldlOptions opt;
-    opt.isearch = isearch_;
-    opt.sigma = sigma_;
-    opt.shift = shift_in;
-    opt.droptol = droptol ? *droptol : 1e-3;
-    opt.mem = mem ? *mem : 0.0;
-    opt.icntl3 = icntl ? icntl[2] : 0;
-    opt.icntl4 = icntl ? icntl[3] : 0;
-    opt.single = icntl ? icntl[4] != 0 : false;
-    ildl = std::make_unique<IldlPrecond>(&ctx);
-    IldlResult r = ildl->setup(n, ia, ja, a, opt);
-    if (r.ierr != 0) {
-        ierr = r.ierr;
-        ildl.reset();
+    pjdinit_ilu(isearch_, sigma_);
+}
+
+// SYMILUPACKSAVEDIAG (src/ilupack/symamgsavediag.c): Gershgorin-type lower
+// estimate used as the default shift.  The index ki ends as the last row
+// (the assignment sits outside the comparison in the reference).
+double Solver::savediag_shift() const {
+    std::vector<double> disc(n, 0.0), dg(n, 0.0);
+    double ti = 0.0;
+    int ki = 0;
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+            int j = ja[k] - 1;
+            if (j == i) {
+                dg[i] = a[k];
+                if (i == 0) ti = a[k];
+                else {
+                    if (a[k] < ti) ti = a[k];
+                    ki = i;
+                }
+            } else {
+                double v = std::fabs(a[k]);
+                disc[i] += v;
+                disc[j] += v;
+            }
+        }
+    double val = ti - disc[ki];
+    for (int i = 0; i < n; ++i) {
+        double t = dg[i] - disc[i];
+        if (t < val && dg[i] - ti < disc[ki]) val = t;
+    }
+    return val;
+}
+
+// PJDINIT, ILU branch (JDaux.F:719-1323): shift choice, factorisation of
+// A - shift0 I, retries with a modified DROPTOL on failure, conversion to an
+// SPD preconditioner (ISEARCH <= 1) and restart with a smaller shift when too
+// many pivots are negative.
+void Solver::pjdinit_ilu(int isearch_, double sigma_) {
+    const double epsmach = 2.220446049250313e-16 * 0.5;
+    const int recompilu = (icntl[1] == 0 || icntl[1] == -2) ? 1 : -1;
+    const double nposeig = (icntl[2] <= 0) ? 0.01 : 0.0;
+    double condest = (timefact > 0.0) ? condest0 : (icntl[3] <= 0 ? 5.0 : (double)icntl[3]);
+    if (*mem <= 0.0) *mem = 10.0;  // default elbow space (symAMGinit.c:278-279)
+    memrequested = *mem;
+    int newsh = 0;
+    for (int attempt = 0; attempt < 32; ++attempt) {
+        auto t0 = std::chrono::steady_clock::now();
+        double dmin = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+                if (ja[k] == i + 1) dmin = (i == 0) ? a[k] : std::min(dmin, a[k]);
+        diagmin = dmin;
+        shift0 = savediag_shift();
+        if (isearch_ >= 2) shift0 = sigma_;
+        int ggsh = 0;
+        if (isearch_ == 1 && *shift < diagmin) shift0 = *shift;
+        else ggsh = 1;
+        if (isearch_ <= 0 && newsh == 1) shift0 = *shift;
+        if (newsh != 1 && timefact <= 0.0) shiftmax = diagmin;
+        IldlOptions opt;
+        opt.droptol = *droptol;
+        opt.condest = condest;
+        opt.mem = *mem;
+        opt.spd = (isearch_ <= 1);
+        opt.single = flagsingle;
+        opt.sweeps = ildl_sweeps;
+        opt.ordering = ildl_ordering;
+        opt.dense_max = ildl_dense_max;
+        IldlHost h;
+        ildl_factor_host(n, ia, ja, a, shift0, opt, h);
+        const IldlResult &r = h.res;
+        memused = r.fill;
+        double totaltime =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (r.ierr == -3 || r.ierr == -2) {
+            if (recompilu == -1 || prvdr == -1) {
+                ierr = r.ierr;
+                return;
+            }
+            prvdr = 1;
+            *droptol = *droptol * 2 * toldiv;
+            continue;
+        } else if (r.ierr != 0) {
+            if (recompilu == -1 || prvdr == 1 || !(memused < slightlyless * memrequested)) {
+                ierr = r.ierr;
+                return;
+            }
+            if (memused < (3 * slightlyless - 2) * memrequested) *droptol /= (4 * toldiv);
+            else if (memused < (2 * slightlyless - 1) * memrequested) *droptol /= (2 * toldiv);
+            else *droptol /= toldiv;
+            *droptol = std::max(*droptol, 1.01 * std::sqrt(epsmach));
+            prvdr = -1;
+            continue;
+        }
+        factspd = 0;
+        if (isearch_ <= 1) {
+            factspd = r.nneg;
+            if (factspd > 0) {
+                if (!(recompilu == -1 || factspd <= n * nposeig)) {
+                    shiftmax = std::min(shift0, shiftmax);
+                    *shift = 2 * *shift - diagmin;
+                    newsh = 1;
+                    prvdr = std::min(prvdr, 0);
+                    continue;
+                }
+            } else if (isearch_ <= 0 || newsh == 1 || ggsh == 1) {
+                factspd = -1;
+            }
+        }
+        ildl = std::make_unique<IldlPrecond>(&ctx);
+        ildl->attach(h, ildl_sweeps);
+        ildl_info = ildl->result;
+        totaltime =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        precond_seconds += totaltime;
+        timefact = std::max(totaltime, 1.0e-2);
+        factdgl = -1;
+        droptol0 = *droptol;
+        condest0 = r.condest;
+        shifts_log.push_back(shift0);
+        ++nsetups;
         return;
     }
-    shift0 = r.shift;
-    diagmin = r.diagmin;
-    shiftmax = r.shiftmax;
-    factspd = r.factspd;
-    if (droptol) *droptol = r.droptol;
-    timefact = r.time;
-    factdgl = -1;
-    shifts_log.push_back(shift0);
-    ++nsetups;
+    ierr = -54321;
 }
 
 // IJOB=2 branch of PJDRVCOM (JDaux.F:282-471)
@@ -310,9 +433,88 @@ void Solver::precond_request(double *z, const double *r) {
             reinit = true;
         }
     }
+    // the ILU case: is a new factorisation cost-effective? (JDaux.F:305-429)
+    const double epsmach = 2.220446049250313e-16 * 0.5;
+    if (factdgl < 0 && !eng.flagcg && (icntl[1] == 0 || icntl[1] == -2) && icntl[0] != 2 &&
+        eng.rcum - rprev <= 6 * std::log(eng.rinit / tol_)) {
+        double tmsol = seconds_since(tic_);
+        if (tmsol < 1e-1) tmsol = 0.0;
+        double time1, time0;
+        if (eng.rcum > rprev) {
+            time1 = tmsol * std::log(eng.rinit / tol_) / (eng.rcum - rprev);
+            time0 = tmsol * std::log(eng.rcurr / tol_) / (eng.rcum - rprev);
+        } else {
+            time1 = 4 * timefact;
+            time0 = 6 * timefact;
+        }
+        const int NE = *neig;
+        if (2 * timefact < time1 && 4 * timefact < (time0 + (nw - NE - 1) * time1)) {
+            auto delta_of = [&]() {
+                return NE > 1 ? std::fabs(eigs[nw - 1] - eigs[nw - NE]) / (double)(NE - 1)
+                              : std::max(*gap_, 0.0);
+            };
+            if (factspd < 0) {
+                if (NE >= 1) {
+                    double delta = delta_of();
+                    tent = std::min(2 * shiftmax - eigs[nw - 1], eigs[nw - 1] - delta);
+                } else {
+                    tent = std::min(2 * shiftmax - eng.shift1,
+                                    eng.shift1 - eng.rcurr - std::max(*gap_, 0.0));
+                    if (2 * eng.rcurr >= diagmin - tent) factspd0 = factspd;
+                }
+                if ((factspd0 == 0 || 4 * timefact < time0) && diagmin > tent &&
+                    2 * (tent - shift0) >= (diagmin - shift0)) {
+                    *shift = tent;
+                    rprev = eng.rcum;
+                    reinit = true;
+                }
+            }
+            if (factspd0 >= 0 && !reinit && memused < slightlyless * memrequested && prvdr <= 0 &&
+                *droptol >= std::sqrt(epsmach)) {
+                if (icntl[1] != -2) reenterfirsttime = false;
+                double dt0 = *droptol;
+                if (!reenterfirsttime) {
+                    if (memused < (3 * slightlyless - 2) * memrequested) *droptol /= (4 * toldiv);
+                    else if (memused < (2 * slightlyless - 1) * memrequested)
+                        *droptol /= (2 * toldiv);
+                    else *droptol /= toldiv;
+                    *droptol = std::max(*droptol, 1.01 * std::sqrt(epsmach));
+                }
+                rprev = eng.rcum;
+                reinit = true;
+                if (*droptol == 1.01 * std::sqrt(epsmach) && dt0 == *droptol && !reenterfirsttime)
+                    reinit = false;
+                reenterfirsttime = false;
+                *shift = shift0;
+                if (factspd > 0) {
+                    if (NE >= 1) {
+                        double delta = delta_of();
+                        *shift = std::min({2 * shift0 - eigs[nw - 1], eigs[nw - 1] - delta,
+                                           shiftmax});
+                    } else if (shift0 > eng.shift1) {
+                        *shift = std::min(eng.shift1, shiftmax);
+                    }
+                }
+            } else if (factspd > 0) {
+                if (NE >= 1) {
+                    if (shift0 > eigs[nw - 1]) {
+                        double delta = delta_of();
+                        *shift = std::min(eigs[nw - 1] - delta, shiftmax);
+                        rprev = eng.rcum;
+                        reinit = true;
+                    }
+                } else if (shift0 > eng.shift1 && 2 * (shift0 - eng.shift1) >= (diagmin - shift0)) {
+                    *shift = std::min(eng.shift1, shiftmax);
+                    rprev = eng.rcum;
+                    reinit = true;
+                }
+            }
+        }
+    }
     if (reinit) {
         pjdinit(std::max(isearch, 1), sigma, *shift);
         if (ierr != 0) throw std::runtime_error("preconditioner setup failed");
+        tic_ = std::chrono::steady_clock::now();
         if (factspd0 < 0 && factspd <= 0) factspd = factspd0;
     }
     if (factdgl > 0) {
@@ -341,11 +543,14 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
     droptol = droptol_;
     mem = mem_;
     icntl = icntl_;
+    tol_ = tol;
+    gap_ = gap ? gap : &gap_dummy_;
     if (*ijob == 0) {
         flagsingle = icntl[4] != 0;
         ierr = 0;
         shifts_log.clear();
         nsetups = 0;
+        precond_seconds = 0.0;
         if ((icntl[1] != -1 && icntl[1] != -2) || factdgl == 0) {
             prvdr = 0;
             timefact = -1.0;
@@ -364,6 +569,7 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
         }
         rprev = 1.0;
         nw = *neig;
+        tic_ = std::chrono::steady_clock::now();
         ops.mv = internal_mv;
         ops.pc = true;
         eng.set_ops(&ops);
@@ -371,10 +577,44 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
     eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
              ndx1, ndx2, iprint, info, gap);
     if (*ijob == 3) *ijob = 1;
-    if (*ijob == 0 && *info > -54321 + 0 && ierr == 0) {
-        // JDaux.F:476-482: suggested SHIFT for a restart
-        *shift = shift0;
-        if (factdgl >= 0 && *neig > 0) *shift = eigs[0];
+    if (*ijob == 0 && ierr == 0) suggest(*neig, eigs, *gap);
+}
+
+// JDaux.F:476-536: suggested SHIFT / DROPTOL for a later restart.
+void Solver::suggest(int NE, const double *ev, double gp) {
+    const double epsmach = 2.220446049250313e-16 * 0.5;
+    *shift = shift0;
+    if (factdgl >= 0) {
+        if (NE > 0) *shift = ev[0];
+    } else if (factspd != 0 || isearch <= 0) {
+        if (NE < 1) return;
+        double delta = NE > 1 ? (ev[NE - 1] - ev[0]) / (double)(NE - 1) : std::max(gp, 0.0);
+        if (factspd > 0) {
+            *shift = std::min(*shift, shiftmax);
+            *shift = std::min(2 * *shift - ev[0], ev[0] - delta);
+        } else {
+            *shift = std::min(2 * shiftmax - ev[0], ev[0] - delta);
+            if (shift0 < ev[0]) *shift = std::max(*shift, shift0);
+        }
+    } else {
+        double tmsol = std::max(seconds_since(tic_), 1.0e-2);
+        double time1 = (eng.rcum > rprev)
+                           ? tmsol * std::log(eng.rinit / tol_) / (eng.rcum - rprev)
+                           : 4 * timefact;
+        if (2 * timefact < time1) {
+            if (memused < slightlyless * memrequested &&
+                (prvdr <= 0 || (isearch <= 1 && *shift < shift0)) &&
+                (isearch >= 2 || *shift <= shift0)) {
+                if (memused < (3 * slightlyless - 2) * memrequested) *droptol /= (4 * toldiv);
+                else if (memused < (2 * slightlyless - 1) * memrequested) *droptol /= (2 * toldiv);
+                else *droptol /= toldiv;
+                *droptol = std::max(*droptol, 1.01 * std::sqrt(epsmach));
+            }
+        } else if (timefact > 2 * time1) {
+            double tent = timefact / time1;
+            *droptol = std::min(tent * *droptol, std::sqrt(*droptol));
+            *droptol = std::max(*droptol, 1.01 * std::sqrt(epsmach));
+        }
     }
 }
 
@@ -422,12 +662,17 @@ void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out) {
     out->spmv_lanes = S.A.lpr;
     for (size_t i = 0; i < S.shifts_log.size() && i < 64; ++i)
         out->setup_shifts[i] = S.shifts_log[i];
+    out->precond_seconds = S.precond_seconds;
+    out->fill = S.ildl_info.fill;
+    out->precond_levels = S.ildl_info.nlev;
+    out->precond_negpiv = S.ildl_info.nneg;
 }
 
 int jdhip_set_matrix(jdhip_ctx *ctx, int n, const int *ia, const int *ja, const double *a,
                      int lanes_per_row) {
     try {
         ctx->S->upload_matrix(n, ia, ja, a, lanes_per_row);
+        ctx->S->A_fp = Solver::fingerprint(n, ia, ja, a);
         return 0;
     } catch (const std::exception &e) {
         set_err(ctx, e.what());
@@ -501,7 +746,12 @@ void jdhip_pjd(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *eigs,
                         std::swap(ja[k], ja[l]);
                         std::swap(a[k], a[l]);
                     }
-            S.upload_matrix(n, ia, ja, a, 0);
+            // re-upload only when the matrix differs from the one held
+            double fp = S.fingerprint(n, ia, ja, a);
+            if (!S.A.rowptr || S.A_n != n || S.A_fp != fp) {
+                S.upload_matrix(n, ia, ja, a, 0);
+                S.A_fp = fp;
+            }
         }
         int keep = icntl[0];
         icntl[0] = 1;
@@ -521,6 +771,159 @@ void jdhip_cleanup(jdhip_ctx *ctx) {
     if (!ctx) return;
     ctx->S->release_precond();
     ctx->S->factdgl = 0;
+}
+
+int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
+    if (!ctx || !name) return -1;
+    Solver &S = *ctx->S;
+    std::string k(name);
+    if (k == "ildl_sweeps") S.ildl_sweeps = (int)value;
+    else if (k == "ildl_ordering") S.ildl_ordering = (int)value;
+    else if (k == "ildl_dense_max") S.ildl_dense_max = (int)value;
+    else if (k == "spmv_lanes") S.A.lpr = (int)value;
+    else return -2;
+    return 0;
+}
+
+int jdhip_precond_apply(jdhip_ctx *ctx, double *z_dev, const double *r_dev) {
+    try {
+        Solver &S = *ctx->S;
+        if (S.factdgl > 0) jdhip::diag_apply(S.ctx.stream, z_dev, S.d_diag, r_dev, S.n);
+        else if (S.ildl && S.ildl->ready()) S.ildl->apply(z_dev, r_dev);
+        else throw std::runtime_error("no preconditioner");
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -1;
+    }
+}
+
+int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels,
+                           long *apply_bytes) {
+    Solver &S = *ctx->S;
+    if (!S.ildl || !S.ildl->ready()) return -1;
+    int f = 0, b = 0;
+    for (int v : S.ildl->result.fwd_levels) f += v;
+    for (int v : S.ildl->result.bwd_levels) b += v;
+    *fwd_levels = f;
+    *bwd_levels = b;
+    *apply_bytes = S.ildl->impl()->apply_bytes();
+    return 0;
+}
+
+}  // extern "C"
+
+struct jdhip_ildl {
+    jdhip::IldlHost h;
+};
+
+namespace {
+template <typename T>
+void export_level(const jdhip::IldlMultilevel<T> &f, int lev, int *perm, double *scal,
+                  int *lb_ptr, int *lb_idx, double *lb_val, int *lc_ptr, int *lc_idx,
+                  double *lc_val, double *d) {
+    const auto &L = f.levels[lev];
+    std::copy(L.perm.begin(), L.perm.end(), perm);
+    for (int i = 0; i < L.n; ++i) scal[i] = (double)L.scal[i];
+    std::copy(L.lb_ptr.begin(), L.lb_ptr.end(), lb_ptr);
+    std::copy(L.lb_idx.begin(), L.lb_idx.end(), lb_idx);
+    for (size_t i = 0; i < L.lb_val.size(); ++i) lb_val[i] = (double)L.lb_val[i];
+    std::copy(L.lc_ptr.begin(), L.lc_ptr.end(), lc_ptr);
+    std::copy(L.lc_idx.begin(), L.lc_idx.end(), lc_idx);
+    for (size_t i = 0; i < L.lc_val.size(); ++i) lc_val[i] = (double)L.lc_val[i];
+    for (int i = 0; i < L.nB; ++i) d[i] = (double)L.d[i];
+}
+}  // namespace
+
+extern "C" {
+
+jdhip_ildl *jdhip_ildl_factor(int n, const int *ia, const int *ja, const double *a, double shift,
+                              double droptol, double condest, int spd, int single, int ordering,
+                              int dense_max, int *ierr) {
+    auto *p = new jdhip_ildl();
+    try {
+        jdhip::IldlOptions opt;
+        opt.droptol = droptol;
+        opt.condest = condest;
+        opt.spd = spd != 0;
+        opt.single = single != 0;
+        opt.ordering = ordering;
+        opt.dense_max = dense_max > 0 ? dense_max : 1500;
+        jdhip::ildl_factor_host(n, ia, ja, a, shift, opt, p->h);
+        *ierr = p->h.res.ierr;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "jadamilu_amd: %s\n", e.what());
+        *ierr = JDHIP_INFO_DEVICE_ERROR;
+    }
+    return p;
+}
+
+void jdhip_ildl_free(jdhip_ildl *f) { delete f; }
+
+int jdhip_ildl_nlevels(jdhip_ildl *f, int *dense_m) {
+    if (f->h.single) {
+        *dense_m = f->h.ff.m;
+        return (int)f->h.ff.levels.size();
+    }
+    *dense_m = f->h.fd.m;
+    return (int)f->h.fd.levels.size();
+}
+
+int jdhip_ildl_level_sizes(jdhip_ildl *f, int lev, int *n, int *nB, long *nnz_lb,
+                           long *nnz_lc) {
+    auto get = [&](const auto &ml) {
+        if (lev < 0 || lev >= (int)ml.levels.size()) return -1;
+        const auto &L = ml.levels[lev];
+        *n = L.n;
+        *nB = L.nB;
+        *nnz_lb = (long)L.lb_idx.size();
+        *nnz_lc = (long)L.lc_idx.size();
+        return 0;
+    };
+    return f->h.single ? get(f->h.ff) : get(f->h.fd);
+}
+
+int jdhip_ildl_level_data(jdhip_ildl *f, int lev, int *perm, double *scal, int *lb_ptr,
+                          int *lb_idx, double *lb_val, int *lc_ptr, int *lc_idx, double *lc_val,
+                          double *d) {
+    if (f->h.single)
+        export_level(f->h.ff, lev, perm, scal, lb_ptr, lb_idx, lb_val, lc_ptr, lc_idx, lc_val, d);
+    else
+        export_level(f->h.fd, lev, perm, scal, lb_ptr, lb_idx, lb_val, lc_ptr, lc_idx, lc_val, d);
+    return 0;
+}
+
+int jdhip_ildl_dense(jdhip_ildl *f, double *minv) {
+    if (f->h.single) {
+        for (size_t i = 0; i < f->h.ff.dinv.size(); ++i) minv[i] = (double)(float)f->h.ff.dinv[i];
+    } else {
+        std::copy(f->h.fd.dinv.begin(), f->h.fd.dinv.end(), minv);
+    }
+    return 0;
+}
+
+int jdhip_ildl_stats(jdhip_ildl *f, double *fill, int *nneg, double *condest, double *seconds) {
+    *fill = f->h.res.fill;
+    *nneg = f->h.res.nneg;
+    *condest = f->h.res.condest;
+    *seconds = f->h.res.seconds;
+    return f->h.res.ierr;
+}
+
+int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
+    try {
+        Solver &S = *ctx->S;
+        S.release_precond();
+        S.ildl = std::make_unique<jdhip::IldlPrecond>(&S.ctx);
+        S.ildl->attach(f->h, sweeps);
+        S.factdgl = -1;
+        S.n = f->h.single ? (f->h.ff.levels.empty() ? f->h.ff.m : f->h.ff.levels[0].n)
+                          : (f->h.fd.levels.empty() ? f->h.fd.m : f->h.fd.levels[0].n);
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -1;
+    }
 }
 
 }  // extern "C"

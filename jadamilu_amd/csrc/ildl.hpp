@@ -1,41 +1,81 @@
-// ildl.hpp -- multilevel incomplete LDL^T preconditioner (host factor, device apply).
+// ildl.hpp -- multilevel incomplete LDL^T preconditioner: host factor
+// (ildl_factor.cpp), device apply (ildl_kernels.hip).
 #pragma once
 #include <memory>
 #include <vector>
 
 #include "engine.hpp"
+#include "ildl_factor.hpp"
 
 namespace jdhip {
 
 struct IldlOptions {
-    int isearch = 0;
-    double sigma = 0.0;
-    double shift = 0.0;
     double droptol = 1e-3;
-    double mem = 0.0;
-    int icntl3 = 0, icntl4 = 0;
-    bool single = false;  // FP32 factor and apply (ICNTL(5) != 0)
+    double condest = 5.0;
+    double mem = 0.0;    // elbow space (stored entries / nnz(A)), <= 0: unlimited
+    bool spd = false;    // |D| conversion (ISEARCH <= 1)
+    bool single = false; // FP32 factor, storage and apply (ICNTL(5) != 0)
+    int sweeps = 0;      // 0: exact level-scheduled sweeps; k > 0: k Jacobi sweeps
+    int ordering = 1;
+    int dense_max = 1500;
 };
 
 struct IldlResult {
     int ierr = 0;
-    double shift = 0.0, diagmin = 0.0, shiftmax = 0.0, droptol = 0.0, time = 0.0;
-    double fill = 0.0;
-    int factspd = 0, nlev = 0;
+    int nlev = 0;
+    long nnz = 0;
+    int nneg = 0;
+    double condest = 0.0;
+    double fill = 0.0;  // stored entries / nnz(upper A)
+    double seconds = 0.0;
+    std::vector<int> level_n, level_nB, fwd_levels, bwd_levels;
 };
 
-struct IldlLevelsDev;
+class IldlImplBase {
+  public:
+    virtual ~IldlImplBase() = default;
+    virtual void apply(double *z, const double *r) = 0;
+    virtual int nlevels() const = 0;
+    virtual int dense_m() const = 0;
+    // level data in double precision (host copies of the factor)
+    virtual void level_sizes(int lev, int *n, int *nB, long *nnz_lb, long *nnz_lc) const = 0;
+    virtual void level_data(int lev, int *perm, double *scal, int *lb_ptr, int *lb_idx,
+                            double *lb_val, int *lc_ptr, int *lc_idx, double *lc_val,
+                            double *d) const = 0;
+    virtual void dense_data(double *minv) const = 0;
+    virtual long apply_bytes() const = 0;  // algorithmic bytes of one apply
+    virtual bool single() const = 0;
+};
+
+// Host factor (no device needed).
+struct IldlHost {
+    bool single = false;
+    IldlMultilevel<double> fd;
+    IldlMultilevel<float> ff;
+    IldlResult res;
+    long nnz_upper = 0;
+};
+
+// Factor (A - shift I), A given as a 1-based upper-triangle CSR.
+void ildl_factor_host(int n, const int *ia, const int *ja, const double *a, double shift,
+                      const IldlOptions &opt, IldlHost &out);
 
 class IldlPrecond {
   public:
-    explicit IldlPrecond(DevCtx *ctx);
-    ~IldlPrecond();
-    IldlResult setup(int n, const int *ia, int *ja, double *a, const IldlOptions &opt);
-    void apply(double *z, const double *r);  // z = M^-1 r on ctx->stream
+    explicit IldlPrecond(DevCtx *ctx) : ctx_(ctx) {}
+    // factor on the host, then upload
+    IldlResult setup(int n, const int *ia, const int *ja, const double *a, double shift,
+                     const IldlOptions &opt);
+    // upload an existing host factor (copied)
+    void attach(const IldlHost &h, int sweeps);
+    void apply(double *z, const double *r) { impl_->apply(z, r); }
+    IldlImplBase *impl() { return impl_.get(); }
+    bool ready() const { return (bool)impl_; }
+    IldlResult result;
 
   private:
     DevCtx *ctx_;
-    std::unique_ptr<IldlLevelsDev> dev_;
+    std::unique_ptr<IldlImplBase> impl_;
 };
 
 }  // namespace jdhip

@@ -1,0 +1,723 @@
+// ildl_factor.cpp -- host multilevel inverse-based incomplete LDL^T (see
+// ildl_factor.hpp for the reference routines each step restates).
+#include "ildl_factor.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <stdexcept>
+
+namespace jdhip {
+
+SymCsr sym_from_upper(int n, const int *ia, const int *ja, const double *a, double shift) {
+    SymCsr S;
+    S.n = n;
+    std::vector<int> cnt(n + 1, 1);  // reserve the diagonal slot
+    cnt[n] = 0;
+    std::vector<int> rc(n, 1);
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+            int j = ja[k] - 1;
+            if (j != i) {
+                rc[i]++;
+                rc[j]++;
+            }
+        }
+    S.ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) S.ptr[i + 1] = S.ptr[i] + rc[i];
+    S.idx.assign(S.ptr[n], 0);
+    S.val.assign(S.ptr[n], 0.0);
+    std::vector<int> fill(S.ptr.begin(), S.ptr.end() - 1);
+    for (int i = 0; i < n; ++i) {
+        S.idx[fill[i]] = i;
+        S.val[fill[i]] = -shift;
+        fill[i]++;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+            int j = ja[k] - 1;
+            if (j == i) {
+                S.val[S.ptr[i]] += a[k];
+            } else {
+                S.idx[fill[i]] = j;
+                S.val[fill[i]++] = a[k];
+                S.idx[fill[j]] = i;
+                S.val[fill[j]++] = a[k];
+            }
+        }
+    // sort each row by column, merging duplicates
+    std::vector<std::pair<int, double>> row;
+    std::vector<int> nptr(n + 1, 0);
+    std::vector<int> nidx;
+    std::vector<double> nval;
+    nidx.reserve(S.idx.size());
+    nval.reserve(S.val.size());
+    for (int i = 0; i < n; ++i) {
+        row.clear();
+        for (int k = S.ptr[i]; k < S.ptr[i + 1]; ++k) row.emplace_back(S.idx[k], S.val[k]);
+        std::sort(row.begin(), row.end(),
+                  [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (size_t t = 0; t < row.size(); ++t) {
+            if (!nidx.empty() && (int)nidx.size() > nptr[i] && nidx.back() == row[t].first)
+                nval.back() += row[t].second;
+            else {
+                nidx.push_back(row[t].first);
+                nval.push_back(row[t].second);
+            }
+        }
+        nptr[i + 1] = (int)nidx.size();
+    }
+    S.ptr.swap(nptr);
+    S.idx.swap(nidx);
+    S.val.swap(nval);
+    return S;
+}
+
+static double pow2_near(double v) {
+    if (!(v > 0.0) || !std::isfinite(v)) return 1.0;
+    return std::exp2(std::round(std::log2(v)));
+}
+
+// Reverse Cuthill-McKee ordering (George & Liu), one BFS per component from
+// a pseudo-peripheral node; returns order[new] = old.
+static std::vector<int> rcm_order(const SymCsr &A) {
+    const int n = A.n;
+    std::vector<int> deg(n), order;
+    order.reserve(n);
+    for (int i = 0; i < n; ++i) deg[i] = A.ptr[i + 1] - A.ptr[i];
+    std::vector<char> seen(n, 0);
+    std::vector<int> lvl(n, -1), q;
+    q.reserve(n);
+    auto bfs_far = [&](int s, int &far, int &depth) {
+        q.clear();
+        q.push_back(s);
+        lvl[s] = 0;
+        size_t h = 0;
+        far = s;
+        depth = 0;
+        while (h < q.size()) {
+            int u = q[h++];
+            if (lvl[u] > depth || (lvl[u] == depth && deg[u] < deg[far])) {
+                depth = lvl[u];
+                far = u;
+            }
+            for (int k = A.ptr[u]; k < A.ptr[u + 1]; ++k) {
+                int v = A.idx[k];
+                if (lvl[v] < 0) {
+                    lvl[v] = lvl[u] + 1;
+                    q.push_back(v);
+                }
+            }
+        }
+        for (int u : q) lvl[u] = -1;
+    };
+    std::vector<int> nb;
+    for (int start = 0; start < n; ++start) {
+        if (seen[start]) continue;
+        // pseudo-peripheral node: repeat BFS from the farthest node
+        int s = start, far, depth, last = -1;
+        for (int it = 0; it < 4; ++it) {
+            bfs_far(s, far, depth);
+            if (depth <= last) break;
+            last = depth;
+            s = far;
+        }
+        size_t head = order.size();
+        order.push_back(s);
+        seen[s] = 1;
+        while (head < order.size()) {
+            int u = order[head++];
+            nb.clear();
+            for (int k = A.ptr[u]; k < A.ptr[u + 1]; ++k) {
+                int v = A.idx[k];
+                if (!seen[v]) {
+                    seen[v] = 1;
+                    nb.push_back(v);
+                }
+            }
+            std::sort(nb.begin(), nb.end(), [&](int x, int y) {
+                return deg[x] < deg[y] || (deg[x] == deg[y] && x < y);
+            });
+            order.insert(order.end(), nb.begin(), nb.end());
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+namespace {
+
+template <typename T>
+struct LevelOut {
+    IldlLevel<T> L;
+    SymCsr S;
+    int nC = 0;
+    int ierr = 0;
+    double maxcond = 1.0;
+};
+
+// One level of the Crout ILDL with deferral (sympiluc.F restated with 1x1
+// pivots).  A is the level's symmetric input matrix (double values).
+template <typename T>
+void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &nnz_stored,
+                  long nnz_limit, LevelOut<T> &out) {
+    const int n = A.n;
+    IldlLevel<T> &L = out.L;
+    L.n = n;
+    // ---- scaling (powers of two) ----
+    std::vector<double> s(n);
+    for (int i = 0; i < n; ++i) {
+        double m = 0.0;
+        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) m = std::max(m, std::fabs(A.val[k]));
+        s[i] = (m > 0.0) ? pow2_near(1.0 / std::sqrt(m)) : 1.0;
+    }
+    // ---- ordering ----
+    std::vector<int> order;
+    if (prm.ordering == 1) order = rcm_order(A);
+    else {
+        order.resize(n);
+        std::iota(order.begin(), order.end(), 0);
+    }
+    std::vector<int> pos(n);
+    for (int k = 0; k < n; ++k) pos[order[k]] = k;
+    // ---- scaled, permuted matrix in processing numbering (CSR, T) ----
+    std::vector<int> ap(n + 1, 0), ai(A.idx.size());
+    std::vector<T> av(A.idx.size());
+    for (int k = 0; k < n; ++k) ap[k + 1] = ap[k] + (A.ptr[order[k] + 1] - A.ptr[order[k]]);
+    {
+        std::vector<std::pair<int, T>> row;
+        for (int k = 0; k < n; ++k) {
+            int o = order[k];
+            row.clear();
+            for (int t = A.ptr[o]; t < A.ptr[o + 1]; ++t) {
+                int j = A.idx[t];
+                row.emplace_back(pos[j], (T)(s[o] * A.val[t] * s[j]));
+            }
+            std::sort(row.begin(), row.end(),
+                      [](const auto &x, const auto &y) { return x.first < y.first; });
+            for (size_t t = 0; t < row.size(); ++t) {
+                ai[ap[k] + t] = row[t].first;
+                av[ap[k] + t] = row[t].second;
+            }
+        }
+    }
+    // ---- Crout loop ----
+    std::vector<int8_t> status(n, 0);  // 0 pending, 1 accepted, 2 deferred
+    std::vector<int> bpos(n, -1), cpos(n, -1);
+    std::vector<T> d(n, T(0)), est(n, T(0));
+    // column arena (accepted columns, rows sorted)
+    std::vector<long> cbeg(n, 0), cend(n, 0), cur(n, 0);
+    std::vector<int> crow;
+    std::vector<T> cval;
+    crow.reserve(ai.size() * 2);
+    cval.reserve(ai.size() * 2);
+    // deferred-row entries per column (linked lists)
+    std::vector<long> dhead(n, -1), dnext;
+    std::vector<int> drow;
+    std::vector<T> dval;
+    // row lists of L: rows -> (column, value)
+    std::vector<long> rhead(n, -1), rtail(n, -1), rnext;
+    std::vector<int> rcol;
+    std::vector<T> rval;
+    rnext.reserve(ai.size() * 2);
+    rcol.reserve(ai.size() * 2);
+    rval.reserve(ai.size() * 2);
+    // sparse accumulator
+    std::vector<T> wv(n, T(0));
+    std::vector<int> wmark(n, -1), wlist;
+    std::vector<std::pair<int, T>> kept;
+    int nB = 0, nC = 0;
+    double maxcond = 1.0;
+    auto acc = [&](int i, T v, int k) {
+        if (wmark[i] != k) {
+            wmark[i] = k;
+            wv[i] = v;
+            wlist.push_back(i);
+        } else {
+            wv[i] += v;
+        }
+    };
+    for (int k = 0; k < n; ++k) {
+        wlist.clear();
+        T wk = T(0);
+        for (int t = ap[k]; t < ap[k + 1]; ++t) {
+            int i = ai[t];
+            if (i == k) wk = av[t];
+            else if (status[i] != 1) acc(i, av[t], k);
+        }
+        for (long e = rhead[k]; e >= 0; e = rnext[e]) {
+            int p = rcol[e];
+            T lkp = rval[e];
+            T coef = lkp * d[p];
+            wk -= coef * lkp;
+            // rows passed so far: deferred ones stay in the column's deferred
+            // list (row k itself is kept ahead of the pointer until its own
+            // accept/defer decision is made)
+            while (cur[p] < cend[p] && crow[cur[p]] < k) {
+                int r = crow[cur[p]];
+                if (status[r] == 2) {
+                    dnext.push_back(dhead[p]);
+                    drow.push_back(r);
+                    dval.push_back(cval[cur[p]]);
+                    dhead[p] = (long)drow.size() - 1;
+                }
+                cur[p]++;
+            }
+            for (long t = cur[p]; t < cend[p]; ++t)
+                if (crow[t] != k) acc(crow[t], -coef * cval[t], k);
+            for (long t = dhead[p]; t >= 0; t = dnext[t])
+                if (drow[t] != k) acc(drow[t], -coef * dval[t], k);
+        }
+        bool skip = !(wk != T(0)) || !std::isfinite((double)wk);
+        T x = T(0);
+        if (!skip) {
+            T xp = T(1) - est[k], xm = T(-1) - est[k];
+            double mup = 0, nup = 0, mum = 0, num = 0;
+            for (int i : wlist) {
+                T li = wv[i] / wk;
+                double a1 = std::fabs((double)(est[i] + xp * li));
+                double a2 = std::fabs((double)(est[i] + xm * li));
+                nup = std::max(nup, a1);
+                mup += a1;
+                num = std::max(num, a2);
+                mum += a2;
+            }
+            double nn;
+            if (mup >= mum) {
+                x = xp;
+                nn = nup;
+            } else {
+                x = xm;
+                nn = num;
+            }
+            if (std::max(std::fabs((double)x), nn) > condest) skip = true;
+        }
+        if (skip) {
+            status[k] = 2;
+            cpos[k] = nC++;
+            continue;
+        }
+        status[k] = 1;
+        bpos[k] = nB++;
+        d[k] = wk;
+        maxcond = std::max(maxcond, std::fabs((double)x));
+        double lmax = 0.0;
+        for (int i : wlist) lmax = std::max(lmax, std::fabs((double)(wv[i] / wk)));
+        const double thr = prm.droptol / maxcond;
+        kept.clear();
+        for (int i : wlist) {
+            T li = wv[i] / wk;
+            if (std::fabs((double)(wk * li)) * lmax > thr) kept.emplace_back(i, li);
+        }
+        std::sort(kept.begin(), kept.end(),
+                  [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
+        cbeg[k] = (long)crow.size();
+        for (auto &kv : kept) {
+            crow.push_back(kv.first);
+            cval.push_back(kv.second);
+            est[kv.first] += x * kv.second;
+            // append to row list of kv.first (keeps column order increasing)
+            rnext.push_back(-1);
+            rcol.push_back(k);
+            rval.push_back(kv.second);
+            long id = (long)rcol.size() - 1;
+            if (rtail[kv.first] < 0) rhead[kv.first] = id;
+            else rnext[rtail[kv.first]] = id;
+            rtail[kv.first] = id;
+        }
+        cend[k] = (long)crow.size();
+        cur[k] = cbeg[k];
+        nnz_stored += (long)kept.size() + 1;
+        if (nnz_limit > 0 && nnz_stored > nnz_limit) {
+            out.ierr = -3;
+            return;
+        }
+    }
+    out.maxcond = maxcond;
+    out.nC = nC;
+    L.nB = nB;
+    // ---- level output ----
+    L.perm.assign(n, 0);
+    for (int k = 0; k < n; ++k) {
+        if (status[k] == 1) L.perm[bpos[k]] = order[k];
+        else L.perm[nB + cpos[k]] = order[k];
+    }
+    L.scal.assign(n, T(1));
+    for (int i = 0; i < n; ++i) L.scal[i] = (T)s[i];
+    L.d.assign(nB, T(0));
+    std::vector<int> lbcnt(nB + 1, 0), lccnt(nC + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        if (status[k] != 1) continue;
+        L.d[bpos[k]] = d[k];
+        for (long t = cbeg[k]; t < cend[k]; ++t) {
+            int r = crow[t];
+            if (status[r] == 1) lbcnt[bpos[r] + 1]++;
+            else lccnt[cpos[r] + 1]++;
+        }
+    }
+    for (int i = 0; i < nB; ++i) lbcnt[i + 1] += lbcnt[i];
+    for (int i = 0; i < nC; ++i) lccnt[i + 1] += lccnt[i];
+    L.lb_ptr = lbcnt;
+    L.lc_ptr = lccnt;
+    L.lb_idx.assign(lbcnt[nB], 0);
+    L.lb_val.assign(lbcnt[nB], T(0));
+    L.lc_idx.assign(lccnt[nC], 0);
+    L.lc_val.assign(lccnt[nC], T(0));
+    std::vector<int> fb(lbcnt.begin(), lbcnt.end() - 1), fc(lccnt.begin(), lccnt.end() - 1);
+    // columns visited in acceptance order -> column indices ascending per row
+    std::vector<int> acc_nodes;
+    acc_nodes.reserve(nB);
+    for (int k = 0; k < n; ++k)
+        if (status[k] == 1) acc_nodes.push_back(k);
+    // L_CB by columns for the Schur complement
+    std::vector<int> lcc_ptr(nB + 1, 0);
+    for (int k : acc_nodes) {
+        int b = bpos[k];
+        int cc = 0;
+        for (long t = cbeg[k]; t < cend[k]; ++t) {
+            int r = crow[t];
+            if (status[r] == 1) {
+                int rr = bpos[r];
+                L.lb_idx[fb[rr]] = b;
+                L.lb_val[fb[rr]++] = cval[t];
+            } else {
+                int rr = cpos[r];
+                L.lc_idx[fc[rr]] = b;
+                L.lc_val[fc[rr]++] = cval[t];
+                ++cc;
+            }
+        }
+        lcc_ptr[b + 1] = cc;
+    }
+    if (nC == 0) return;
+    for (int b = 0; b < nB; ++b) lcc_ptr[b + 1] += lcc_ptr[b];
+    std::vector<int> lcc_idx(lcc_ptr[nB]);
+    std::vector<T> lcc_val(lcc_ptr[nB]);
+    {
+        std::vector<int> f(lcc_ptr.begin(), lcc_ptr.end() - 1);
+        for (int c = 0; c < nC; ++c)
+            for (int t = L.lc_ptr[c]; t < L.lc_ptr[c + 1]; ++t) {
+                int b = L.lc_idx[t];
+                lcc_idx[f[b]] = c;
+                lcc_val[f[b]++] = L.lc_val[t];
+            }
+    }
+    // ---- Schur complement S = A_CC - L_CB D_B L_CB^T with dropping ----
+    std::vector<int> cnode(nC);
+    for (int k = 0; k < n; ++k)
+        if (status[k] == 2) cnode[cpos[k]] = k;
+    SymCsr &S = out.S;
+    S.n = nC;
+    S.ptr.assign(nC + 1, 0);
+    std::vector<double> sv(nC, 0.0);
+    std::vector<int> smark(nC, -1), slist;
+    const double sdrop = prm.droptol;
+    std::vector<std::pair<int, double>> srow;
+    for (int c = 0; c < nC; ++c) {
+        slist.clear();
+        int k = cnode[c];
+        auto sacc = [&](int j, double v) {
+            if (smark[j] != c) {
+                smark[j] = c;
+                sv[j] = v;
+                slist.push_back(j);
+            } else {
+                sv[j] += v;
+            }
+        };
+        for (int t = ap[k]; t < ap[k + 1]; ++t) {
+            int j = ai[t];
+            if (status[j] == 2) sacc(cpos[j], (double)av[t]);
+        }
+        sacc(c, 0.0);
+        for (int t = L.lc_ptr[c]; t < L.lc_ptr[c + 1]; ++t) {
+            int b = L.lc_idx[t];
+            double coef = (double)L.lc_val[t] * (double)L.d[b];
+            for (int u = lcc_ptr[b]; u < lcc_ptr[b + 1]; ++u)
+                sacc(lcc_idx[u], -coef * (double)lcc_val[u]);
+        }
+        srow.clear();
+        for (int j : slist)
+            if (j == c || std::fabs(sv[j]) > sdrop) srow.emplace_back(j, sv[j]);
+        std::sort(srow.begin(), srow.end(),
+                  [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
+        S.ptr[c + 1] = S.ptr[c] + (int)srow.size();
+        for (auto &e : srow) {
+            S.idx.push_back(e.first);
+            S.val.push_back(e.second);
+        }
+    }
+    // symmetrise the pattern and values: S := (S + S^T) / 2 on the union
+    {
+        std::vector<int> tp(nC + 1, 0);
+        for (int c = 0; c < nC; ++c)
+            for (int t = S.ptr[c]; t < S.ptr[c + 1]; ++t) tp[S.idx[t] + 1]++;
+        for (int c = 0; c < nC; ++c) tp[c + 1] += tp[c];
+        std::vector<int> ti(tp[nC]);
+        std::vector<double> tv(tp[nC]);
+        std::vector<int> f(tp.begin(), tp.end() - 1);
+        for (int c = 0; c < nC; ++c)
+            for (int t = S.ptr[c]; t < S.ptr[c + 1]; ++t) {
+                int j = S.idx[t];
+                ti[f[j]] = c;
+                tv[f[j]++] = S.val[t];
+            }
+        SymCsr U;
+        U.n = nC;
+        U.ptr.assign(nC + 1, 0);
+        for (int c = 0; c < nC; ++c) {
+            int a = S.ptr[c], ae = S.ptr[c + 1], b = tp[c], be = tp[c + 1];
+            while (a < ae || b < be) {
+                int ja_ = (a < ae) ? S.idx[a] : INT32_MAX;
+                int jb_ = (b < be) ? ti[b] : INT32_MAX;
+                double v;
+                int j;
+                if (ja_ == jb_) {
+                    j = ja_;
+                    v = 0.5 * (S.val[a++] + tv[b++]);
+                } else if (ja_ < jb_) {
+                    j = ja_;
+                    v = 0.5 * S.val[a++];
+                } else {
+                    j = jb_;
+                    v = 0.5 * tv[b++];
+                }
+                U.idx.push_back(j);
+                U.val.push_back(v);
+            }
+            U.ptr[c + 1] = (int)U.idx.size();
+        }
+        S = std::move(U);
+    }
+}
+
+// Dense LDL^T with Bunch-Kaufman pivoting (alpha = (1+sqrt(17))/8) of the
+// final level; returns the explicit inverse of P L Dc L^T P^T where Dc is D
+// (indefinite) or |D| (SPD conversion, blockwise |D| for 2x2 pivots).
+int dense_inverse(const SymCsr &S, bool spd, std::vector<double> &inv, int &nneg) {
+    const int m = S.n;
+    std::vector<double> A((size_t)m * m, 0.0);
+    auto Ar = [&](int i, int j) -> double & { return A[(size_t)i * m + j]; };
+    for (int i = 0; i < m; ++i)
+        for (int t = S.ptr[i]; t < S.ptr[i + 1]; ++t) Ar(i, S.idx[t]) = S.val[t];
+    std::vector<int> perm(m);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::vector<int> bsize(m, 0);  // 1 or 2 at block starts
+    const double alpha = (1.0 + std::sqrt(17.0)) / 8.0;
+    auto sym_swap = [&](int p, int q) {
+        if (p == q) return;
+        for (int j = 0; j < m; ++j) std::swap(Ar(p, j), Ar(q, j));
+        for (int i = 0; i < m; ++i) std::swap(Ar(i, p), Ar(i, q));
+        std::swap(perm[p], perm[q]);
+    };
+    int k = 0;
+    while (k < m) {
+        double akk = std::fabs(Ar(k, k)), colmax = 0.0;
+        int imax = k;
+        for (int i = k + 1; i < m; ++i)
+            if (std::fabs(Ar(i, k)) > colmax) {
+                colmax = std::fabs(Ar(i, k));
+                imax = i;
+            }
+        int kstep = 1, kp = k;
+        if (std::max(akk, colmax) == 0.0) return k + 1;
+        if (akk < alpha * colmax) {
+            double rowmax = 0.0;
+            for (int j = k; j < m; ++j)
+                if (j != imax) rowmax = std::max(rowmax, std::fabs(Ar(imax, j)));
+            if (akk >= alpha * colmax * (colmax / rowmax)) kp = k;
+            else if (std::fabs(Ar(imax, imax)) >= alpha * rowmax) kp = imax;
+            else {
+                kp = imax;
+                kstep = 2;
+            }
+        }
+        int kk = k + kstep - 1;
+        sym_swap(kk, kp);
+        if (kstep == 1) {
+            double dk = Ar(k, k);
+            for (int i = k + 1; i < m; ++i) Ar(i, k) /= dk;
+            for (int j = k + 1; j < m; ++j) {
+                double f = Ar(j, k) * dk;
+                if (f == 0.0) continue;
+                for (int i = k + 1; i < m; ++i) Ar(i, j) -= Ar(i, k) * f;
+            }
+            for (int j = k + 1; j < m; ++j) Ar(k, j) = 0.0;
+            bsize[k] = 1;
+        } else {
+            double d11 = Ar(k, k), d21 = Ar(k + 1, k), d22 = Ar(k + 1, k + 1);
+            double det = d11 * d22 - d21 * d21;
+            // L(:,k:k+1) = A(:,k:k+1) D^-1
+            for (int i = k + 2; i < m; ++i) {
+                double a1 = Ar(i, k), a2 = Ar(i, k + 1);
+                double l1 = (a1 * d22 - a2 * d21) / det;
+                double l2 = (a2 * d11 - a1 * d21) / det;
+                Ar(i, k) = l1;
+                Ar(i, k + 1) = l2;
+            }
+            for (int j = k + 2; j < m; ++j) {
+                double f1 = Ar(j, k) * d11 + Ar(j, k + 1) * d21;
+                double f2 = Ar(j, k) * d21 + Ar(j, k + 1) * d22;
+                for (int i = k + 2; i < m; ++i) Ar(i, j) -= Ar(i, k) * f1 + Ar(i, k + 1) * f2;
+            }
+            for (int j = k + 2; j < m; ++j) Ar(k, j) = Ar(k + 1, j) = 0.0;
+            Ar(k, k + 1) = d21;
+            bsize[k] = 2;
+            bsize[k + 1] = 0;
+        }
+        k += kstep;
+    }
+    // D blocks (possibly |D|) inverted
+    nneg = 0;
+    std::vector<double> Dinv((size_t)m * 2, 0.0);  // per row: (self, partner) entries
+    for (k = 0; k < m;) {
+        if (bsize[k] == 1) {
+            double dk = Ar(k, k);
+            if (dk < 0) ++nneg;
+            if (spd) dk = std::fabs(dk);
+            Dinv[2 * k] = 1.0 / dk;
+            ++k;
+        } else {
+            double a = Ar(k, k), b = Ar(k + 1, k), c = Ar(k + 1, k + 1);
+            // eigen-decomposition of [[a b][b c]]
+            double tr = 0.5 * (a + c), df = 0.5 * (a - c);
+            double rt = std::sqrt(df * df + b * b);
+            double l1 = tr + rt, l2 = tr - rt;
+            nneg += (l1 < 0) + (l2 < 0);
+            if (spd) {
+                double th = 0.5 * std::atan2(2 * b, a - c);
+                double cs = std::cos(th), sn = std::sin(th);
+                double e1 = std::fabs(l1), e2 = std::fabs(l2);
+                a = cs * cs * e1 + sn * sn * e2;
+                c = sn * sn * e1 + cs * cs * e2;
+                b = cs * sn * (e1 - e2);
+            }
+            double det = a * c - b * b;
+            Dinv[2 * k] = c / det;
+            Dinv[2 * k + 1] = -b / det;
+            Dinv[2 * (k + 1)] = a / det;
+            Dinv[2 * (k + 1) + 1] = -b / det;
+            k += 2;
+        }
+    }
+    // W = L^-1 (unit lower), then inv_perm = W^T Dinv W
+    std::vector<double> W((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j) {  // column j of L^-1
+        std::vector<double> col(m, 0.0);
+        col[j] = 1.0;
+        for (int i = j; i < m; ++i) {
+            double v = col[i];
+            if (v == 0.0) continue;
+            int ilo = i + 1;
+            if (bsize[i] == 2) ilo = i + 2;  // L(i+1,i) is zero inside a 2x2 block
+            for (int r = ilo; r < m; ++r) col[r] -= Ar(r, i) * v;
+        }
+        for (int i = 0; i < m; ++i) W[(size_t)i * m + j] = col[i];
+    }
+    std::vector<double> DW((size_t)m * m, 0.0);
+    for (k = 0; k < m;) {
+        if (bsize[k] == 1) {
+            for (int j = 0; j < m; ++j) DW[(size_t)k * m + j] = Dinv[2 * k] * W[(size_t)k * m + j];
+            ++k;
+        } else {
+            for (int j = 0; j < m; ++j) {
+                double w1 = W[(size_t)k * m + j], w2 = W[(size_t)(k + 1) * m + j];
+                DW[(size_t)k * m + j] = Dinv[2 * k] * w1 + Dinv[2 * k + 1] * w2;
+                DW[(size_t)(k + 1) * m + j] = Dinv[2 * (k + 1) + 1] * w1 + Dinv[2 * (k + 1)] * w2;
+            }
+            k += 2;
+        }
+    }
+    inv.assign((size_t)m * m, 0.0);
+    std::vector<double> tmp((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i)
+        for (int r = 0; r < m; ++r) {
+            double w = W[(size_t)r * m + i];
+            if (w == 0.0) continue;
+            const double *dw = &DW[(size_t)r * m];
+            double *t = &tmp[(size_t)i * m];
+            for (int j = 0; j < m; ++j) t[j] += w * dw[j];
+        }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) inv[(size_t)perm[i] * m + perm[j]] = tmp[(size_t)i * m + j];
+    return 0;
+}
+
+}  // namespace
+
+template <typename T>
+IldlStats ildl_factor(const SymCsr &A0, const IldlParams &prm, IldlMultilevel<T> &out,
+                      long nnz_upper_ref) {
+    IldlStats st;
+    out = IldlMultilevel<T>();
+    long nnz_stored = 0;
+    long limit = (prm.mem > 0.0) ? (long)(prm.mem * (double)nnz_upper_ref) : 0;
+    SymCsr cur = A0;
+    double condest = prm.condest;
+    st.condest_used = condest;
+    int nneg = 0;
+    for (int lev = 0; lev < prm.max_levels && cur.n > 0; ++lev) {
+        // dense switch (symAMGsetup.c:450): the level is dense enough that a
+        // packed factor (n(n+1)/2) costs less than 3 x its sparse storage
+        const double full = 0.5 * (double)cur.n * (cur.n + 1.0);
+        if (cur.n <= 64 ||
+            (cur.n <= prm.dense_max && 3.0 * (double)cur.idx.size() >= full))
+            break;
+        LevelOut<T> lo;
+        double ce = condest;
+        long saved = nnz_stored;
+        for (int tries = 0;; ++tries) {
+            nnz_stored = saved;
+            lo = LevelOut<T>();
+            factor_level<T>(cur, prm, ce, nnz_stored, limit, lo);
+            if (lo.ierr) {
+                st.ierr = lo.ierr;
+                return st;
+            }
+            if (lo.nC <= 0.75 * cur.n || tries >= 4) break;
+            ce *= 2.0;
+        }
+        st.condest_used = std::max(st.condest_used, ce);
+        st.level_n.push_back(cur.n);
+        st.level_nB.push_back(lo.L.nB);
+        for (auto v : lo.L.d) nneg += (v < T(0));
+        out.levels.push_back(std::move(lo.L));
+        if (lo.nC == 0) {
+            cur = SymCsr();
+            break;
+        }
+        cur = std::move(lo.S);
+    }
+    if (cur.n > 0) {
+        if (cur.n > 8000) {
+            st.ierr = -3;
+            return st;
+        }
+        int dn = 0;
+        int info = dense_inverse(cur, prm.spd, out.dinv, dn);
+        if (info) {
+            st.ierr = info;
+            return st;
+        }
+        out.m = cur.n;
+        nneg += dn;
+        nnz_stored += (long)cur.n * (cur.n + 1) / 2;
+    }
+    if (prm.spd)
+        for (auto &L : out.levels)
+            for (auto &v : L.d) v = (v < T(0)) ? -v : v;
+    out.nneg = nneg;
+    out.nnz_factor = nnz_stored;
+    st.nlev = (int)out.levels.size() + (out.m > 0 ? 1 : 0);
+    st.nnz = nnz_stored;
+    st.nneg = nneg;
+    return st;
+}
+
+template IldlStats ildl_factor<double>(const SymCsr &, const IldlParams &,
+                                       IldlMultilevel<double> &, long);
+template IldlStats ildl_factor<float>(const SymCsr &, const IldlParams &,
+                                      IldlMultilevel<float> &, long);
+
+}  // namespace jdhip

@@ -1,0 +1,177 @@
+// ildl_kernels.hip -- gfx950 kernels of the multilevel ILDL apply
+// (see ildl_kernels.hpp).  Rows of the incomplete factor hold a handful of
+// entries, so a row is served by a group of 4 lanes (16 rows per wave64);
+// every loop is grid-stride and every read of a vector entry is a gather
+// through the CSR column index, coalesced along the row.
+#include "ildl_kernels.hpp"
+
+#include <algorithm>
+
+namespace jdhip {
+
+static constexpr int kB = 256;
+static constexpr int kLpr = 4;
+
+static inline int blocks_for(long n, int per_block, int cap = 8192) {
+    long b = (n + per_block - 1) / per_block;
+    return (int)std::max<long>(1, std::min<long>(b, cap));
+}
+
+template <typename TI, typename T>
+__global__ __launch_bounds__(kB) void k_gather(int n, const int *__restrict__ perm,
+                                               const T *__restrict__ scal,
+                                               const TI *__restrict__ in, T *__restrict__ buf) {
+    for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
+        int p = perm[i];
+        buf[i] = scal[p] * (T)in[p];
+    }
+}
+
+template <typename T, typename TO>
+__global__ __launch_bounds__(kB) void k_scatter(int n, int nB, const int *__restrict__ perm,
+                                                const T *__restrict__ scal,
+                                                const T *__restrict__ buf,
+                                                const T *__restrict__ zc, TO *__restrict__ out) {
+    for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
+        int p = perm[i];
+        T v = (i < nB) ? buf[i] : zc[i - nB];
+        out[p] = (TO)(scal[p] * v);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows, int count,
+                                                   const int *__restrict__ ptr,
+                                                   const int *__restrict__ idx,
+                                                   const T *__restrict__ val, T *x) {
+    const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
+    for (long t = (long)blockIdx.x * (kB / kLpr) + g; t < count;
+         t += (long)gridDim.x * (kB / kLpr)) {
+        const int r = rows[t];
+        T s = T(0);
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += val[k] * x[idx[k]];
+#pragma unroll
+        for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
+        if (l == 0) x[r] -= s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_jacobi(int nrows, const int *__restrict__ ptr,
+                                               const int *__restrict__ idx,
+                                               const T *__restrict__ val,
+                                               const T *__restrict__ b,
+                                               const T *__restrict__ xin, T *__restrict__ xout) {
+    const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
+    for (long r = (long)blockIdx.x * (kB / kLpr) + g; r < nrows;
+         r += (long)gridDim.x * (kB / kLpr)) {
+        T s = T(0);
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += val[k] * xin[idx[k]];
+#pragma unroll
+        for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
+        if (l == 0) xout[r] = b[r] - s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_spmv_sub(int nrows, const int *__restrict__ ptr,
+                                                 const int *__restrict__ idx,
+                                                 const T *__restrict__ val,
+                                                 const T *__restrict__ x, T *__restrict__ y) {
+    const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
+    for (long r = (long)blockIdx.x * (kB / kLpr) + g; r < nrows;
+         r += (long)gridDim.x * (kB / kLpr)) {
+        T s = T(0);
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += val[k] * x[idx[k]];
+#pragma unroll
+        for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
+        if (l == 0) y[r] -= s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_dscale(int n, const T *__restrict__ dinv, T *x) {
+    for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
+        x[i] *= dinv[i];
+}
+
+// one wave per row of the dense inverse
+template <typename T>
+__global__ __launch_bounds__(kB) void k_dense_gemv(int m, const T *__restrict__ Minv,
+                                                   const T *__restrict__ x, T *__restrict__ y) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (long r = (long)blockIdx.x * (kB / 64) + w; r < m; r += (long)gridDim.x * (kB / 64)) {
+        const T *row = Minv + r * (long)m;
+        T s = T(0);
+        for (int j = lane; j < m; j += 64) s += row[j] * x[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) y[r] = s;
+    }
+}
+
+template <typename TI, typename T>
+void ildl_gather(hipStream_t s, int n, const int *perm, const T *scal, const TI *in, T *buf) {
+    if (n <= 0) return;
+    k_gather<TI, T><<<blocks_for(n, kB), kB, 0, s>>>(n, perm, scal, in, buf);
+}
+template <typename T, typename TO>
+void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const T *scal, const T *buf,
+                  const T *zc, TO *out) {
+    if (n <= 0) return;
+    k_scatter<T, TO><<<blocks_for(n, kB), kB, 0, s>>>(n, nB, perm, scal, buf, zc, out);
+}
+template <typename T>
+void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
+                     const T *val, T *x) {
+    if (count <= 0) return;
+    k_trsv_level<T><<<blocks_for(count, kB / kLpr), kB, 0, s>>>(rows, count, ptr, idx, val, x);
+}
+template <typename T>
+void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
+                       const T *b, const T *xin, T *xout) {
+    if (nrows <= 0) return;
+    k_jacobi<T><<<blocks_for(nrows, kB / kLpr), kB, 0, s>>>(nrows, ptr, idx, val, b, xin, xout);
+}
+template <typename T>
+void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
+                   const T *x, T *y) {
+    if (nrows <= 0) return;
+    k_spmv_sub<T><<<blocks_for(nrows, kB / kLpr), kB, 0, s>>>(nrows, ptr, idx, val, x, y);
+}
+template <typename T>
+void ildl_dscale(hipStream_t s, int n, const T *dinv, T *x) {
+    if (n <= 0) return;
+    k_dscale<T><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
+}
+template <typename T>
+void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
+    if (m <= 0) return;
+    k_dense_gemv<T><<<blocks_for(m, kB / 64, 4096), kB, 0, s>>>(m, Minv, x, y);
+}
+
+#define INST(T)                                                                              \
+    template void ildl_trsv_level<T>(hipStream_t, const int *, int, const int *, const int *, \
+                                     const T *, T *);                                        \
+    template void ildl_jacobi_sweep<T>(hipStream_t, int, const int *, const int *, const T *, \
+                                       const T *, const T *, T *);                           \
+    template void ildl_spmv_sub<T>(hipStream_t, int, const int *, const int *, const T *,     \
+                                   const T *, T *);                                          \
+    template void ildl_dscale<T>(hipStream_t, int, const T *, T *);                          \
+    template void ildl_dense_gemv<T>(hipStream_t, int, const T *, const T *, T *);
+INST(double)
+INST(float)
+template void ildl_gather<double, double>(hipStream_t, int, const int *, const double *,
+                                          const double *, double *);
+template void ildl_gather<double, float>(hipStream_t, int, const int *, const float *,
+                                         const double *, float *);
+template void ildl_gather<float, float>(hipStream_t, int, const int *, const float *,
+                                        const float *, float *);
+template void ildl_scatter<double, double>(hipStream_t, int, int, const int *, const double *,
+                                           const double *, const double *, double *);
+template void ildl_scatter<float, double>(hipStream_t, int, int, const int *, const float *,
+                                          const float *, const float *, double *);
+template void ildl_scatter<float, float>(hipStream_t, int, int, const int *, const float *,
+                                         const float *, const float *, float *);
+
+}  // namespace jdhip

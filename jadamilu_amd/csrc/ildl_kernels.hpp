@@ -1,0 +1,38 @@
+// ildl_kernels.hpp -- gfx950 kernels of the multilevel ILDL apply.
+//
+// Per level (src/ilupack/symAMGsol.c:173-628 restated):
+//   gather+scale -> forward L_B sweep -> coarse rhs (C rows) -> D^-1 ->
+//   [next level] -> B rows minus L_CB^T z_C -> backward L_B^T sweep ->
+//   scatter+scale.
+// The triangular sweeps are level-scheduled (one launch per dependency
+// level, rows of a level in parallel) or, in sweep mode, a fixed number of
+// Jacobi iterations x <- b - N x on the strictly triangular part.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace jdhip {
+
+template <typename TI, typename T>
+void ildl_gather(hipStream_t s, int n, const int *perm, const T *scal, const TI *in, T *buf);
+template <typename T, typename TO>
+void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const T *scal, const T *buf,
+                  const T *zc, TO *out);
+// x[rows of one level] -= sum_j M(row, j) x[j]   (unit-diagonal triangular)
+template <typename T>
+void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
+                     const T *val, T *x);
+// xout = b - M xin over nrows rows (one Jacobi sweep of the unit triangle)
+template <typename T>
+void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
+                       const T *b, const T *xin, T *xout);
+// y[r] -= sum_j M(r, j) x[j] for r < nrows, then optional y[r] *= dscale[r]
+template <typename T>
+void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
+                   const T *x, T *y);
+template <typename T>
+void ildl_dscale(hipStream_t s, int n, const T *dinv, T *x);
+// y = Minv x, Minv m x m row-major
+template <typename T>
+void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y);
+
+}  // namespace jdhip

@@ -181,10 +181,34 @@ class Context:
         self._L.jdhip_get_stats(self.h, C.byref(s))
         return {"inner_iters": s.inner_iters, "host_syncs": s.host_syncs,
                 "n_precond_setups": s.n_precond_setups, "spmv_lanes": s.spmv_lanes,
-                "setup_shifts": list(s.setup_shifts[: s.n_precond_setups])}
+                "setup_shifts": list(s.setup_shifts[: s.n_precond_setups]),
+                "precond_seconds": s.precond_seconds, "fill": s.fill,
+                "precond_levels": s.precond_levels, "precond_negpiv": s.precond_negpiv}
 
     def stream_ptr(self):
         return self._L.jdhip_stream(self.h)
+
+    def set_option(self, name, value):
+        rc = self._L.jdhip_set_option(self.h, name.encode(), float(value))
+        if rc != 0:
+            raise ValueError(f"unknown option {name}")
+
+    def attach_ildl(self, factor, sweeps=0):
+        """Upload an IldlFactor (jadamilu_amd.ildl) as the context's preconditioner."""
+        rc = self._L.jdhip_ildl_attach(self.h, factor.h, sweeps)
+        if rc != 0:
+            raise RuntimeError(self.error())
+
+    def precond_apply(self, z_ptr, r_ptr):
+        rc = self._L.jdhip_precond_apply(self.h, C.c_void_p(z_ptr), C.c_void_p(r_ptr))
+        if rc != 0:
+            raise RuntimeError(self.error())
+
+    def ildl_device_info(self):
+        f, b, nb = C.c_int(), C.c_int(), C.c_long()
+        if self._L.jdhip_ildl_device_info(self.h, C.byref(f), C.byref(b), C.byref(nb)) != 0:
+            return None
+        return {"fwd_levels": f.value, "bwd_levels": b.value, "apply_bytes": nb.value}
 
     def set_matrix(self, ia, ja, a, lanes_per_row=0):
         ia = np.ascontiguousarray(ia, dtype=np.int32)

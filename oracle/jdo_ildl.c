@@ -1,0 +1,96 @@
+/*
+ * jdo_ildl.c -- CPU oracle of the multilevel ILDL apply.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Sequential restatement of the multilevel
+ * forward/backward substitution of DSYMAMGsol (src/ilupack/symAMGsol.c:
+ * 110-631) for 1x1 pivots: per level scale+permute, forward sweep with the
+ * unit lower factor L_B (sympiluclsol.F), coarse right-hand side with L_CB
+ * (CSRmatvec.c), D^-1, recursion on the Schur complement, L_CB^T product
+ * (CSRmattvec.c), backward sweep with L_B^T (sympilucusol.F), unpermute +
+ * scale; the final level is an explicit dense inverse.  It consumes the
+ * factor exported through the product's jdhip_ildl_level_data().
+ */
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    int n, nB;
+    const int *perm;
+    const double *scal;
+    const int *lb_ptr, *lb_idx;
+    const double *lb_val;
+    const int *lc_ptr, *lc_idx;
+    const double *lc_val;
+    const double *d;
+} jdo_ildl_level;
+
+static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const double *minv,
+                        const double *in, double *out)
+{
+    if (l == nlev) {
+        for (int i = 0; i < m; i++) {
+            double s = 0.0;
+            for (int j = 0; j < m; j++) s += minv[(long)i * m + j] * in[j];
+            out[i] = s;
+        }
+        return;
+    }
+    const jdo_ildl_level *V = &L[l];
+    const int n = V->n, nB = V->nB, nC = n - nB;
+    double *buf = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double *zc = (double *)malloc(sizeof(double) * (nC > 0 ? nC : 1));
+    for (int i = 0; i < n; i++) {
+        int p = V->perm[i];
+        buf[i] = V->scal[p] * in[p];
+    }
+    /* forward: L_B y = x_B (rows ascending, entries j < i) */
+    for (int i = 0; i < nB; i++) {
+        double s = 0.0;
+        for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) s += V->lb_val[k] * buf[V->lb_idx[k]];
+        buf[i] -= s;
+    }
+    /* coarse right-hand side: x_C - L_CB y */
+    for (int c = 0; c < nC; c++) {
+        double s = 0.0;
+        for (int k = V->lc_ptr[c]; k < V->lc_ptr[c + 1]; k++) s += V->lc_val[k] * buf[V->lc_idx[k]];
+        buf[nB + c] -= s;
+    }
+    for (int b = 0; b < nB; b++) buf[b] /= V->d[b];
+    if (nC > 0) level_apply(l + 1, nlev, L, m, minv, buf + nB, zc);
+    /* B rows minus L_CB^T z_C */
+    for (int c = 0; c < nC; c++)
+        for (int k = V->lc_ptr[c]; k < V->lc_ptr[c + 1]; k++)
+            buf[V->lc_idx[k]] -= V->lc_val[k] * zc[c];
+    /* backward: L_B^T z = t (column-oriented over the rows of L_B) */
+    for (int i = nB - 1; i >= 0; i--) {
+        double zi = buf[i];
+        for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) buf[V->lb_idx[k]] -= V->lb_val[k] * zi;
+    }
+    for (int i = 0; i < n; i++) {
+        int p = V->perm[i];
+        out[p] = V->scal[p] * (i < nB ? buf[i] : zc[i - nB]);
+    }
+    free(buf);
+    free(zc);
+}
+
+void jdo_ildl_apply(int nlev, const jdo_ildl_level *levels, int m, const double *minv,
+                    const double *r, double *z)
+{
+    level_apply(0, nlev, levels, m, minv, r, z);
+}
+
+/* Adapter with the jdo_psolve_fn signature (see jd_oracle.h). */
+typedef struct {
+    int nlev;
+    const jdo_ildl_level *levels;
+    int m;
+    const double *minv;
+} jdo_ildl_ctx;
+
+void jdo_ildl_psolve(void *ctx, int n, double *out, const double *in)
+{
+    const jdo_ildl_ctx *c = (const jdo_ildl_ctx *)ctx;
+    (void)n;
+    level_apply(0, c->nlev, c->levels, c->m, c->minv, in, out);
+}

@@ -56,6 +56,8 @@ def lib():
         L.jdo_sygv.argtypes = [C.c_int, _d, C.c_int, _d, C.c_int, _d]
         L.jdo_sortrx.argtypes = [C.c_int, _d, _i]
         L.jdo_cleanup.argtypes = [C.c_void_p]
+        L.jdo_ildl_apply.argtypes = [C.c_int, C.c_void_p, C.c_int, _d, _d, _d]
+        L.jdo_set_psolve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -172,6 +174,110 @@ def matvec_upper(ia, ja, a):
                          _p(xi, C.c_double), _p(xo, C.c_double))
         xout[:] = xo
     return mv
+
+
+class _Level(C.Structure):
+    _fields_ = [("n", C.c_int), ("nB", C.c_int), ("perm", _i), ("scal", _d),
+                ("lb_ptr", _i), ("lb_idx", _i), ("lb_val", _d),
+                ("lc_ptr", _i), ("lc_idx", _i), ("lc_val", _d), ("d", _d)]
+
+
+class _IldlCtx(C.Structure):
+    _fields_ = [("nlev", C.c_int), ("levels", C.c_void_p), ("m", C.c_int), ("minv", _d)]
+
+
+def _levels_struct(factor):
+    levs = factor["levels"]
+    arr = (_Level * max(len(levs), 1))()
+    keep = [arr]
+    for k, lv in enumerate(levs):
+        fields = {}
+        for name in ("perm", "lb_ptr", "lb_idx", "lc_ptr", "lc_idx"):
+            a = np.ascontiguousarray(lv[name], dtype=np.int32)
+            if a.size == 0:
+                a = np.zeros(1, np.int32)
+            keep.append(a)
+            fields[name] = a.ctypes.data_as(_i)
+        for name in ("scal", "lb_val", "lc_val", "d"):
+            a = np.ascontiguousarray(lv[name], dtype=np.float64)
+            if a.size == 0:
+                a = np.zeros(1)
+            keep.append(a)
+            fields[name] = a.ctypes.data_as(_d)
+        arr[k] = _Level(lv["n"], lv["nB"], **fields)
+    minv = np.ascontiguousarray(factor["minv"], dtype=np.float64).ravel()
+    if minv.size == 0:
+        minv = np.zeros(1)
+    keep.append(minv)
+    return arr, minv, keep
+
+
+def pjd_ildl(n, ia, ja, a, factor, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20,
+             maxiter=1000, tol=1e-10, icntl=(0, 1, 0, 0, 0), x0=None):
+    """DPJD of the oracle (internal MATVECA) preconditioned by the oracle's
+    multilevel apply of an exported factor (the factor is computed once,
+    outside; the oracle has no adaptive re-factorisation)."""
+    L = lib()
+    arr, minv, keep = _levels_struct(factor)
+    ctx = _IldlCtx(len(factor["levels"]), C.cast(arr, C.c_void_p), factor["m"],
+                   minv.ctypes.data_as(_d))
+    s = L.jdo_create()
+    try:
+        fn = C.cast(L.jdo_ildl_psolve, C.c_void_p)
+        L.jdo_set_psolve(s, fn, C.byref(ctx))
+        lx = lx_optimal(n, neig, madspace)
+        x = np.zeros(lx)
+        if x0 is not None:
+            x[: x0.size] = x0.ravel(order="F")
+        aa = np.array(a, dtype=np.float64)
+        jj = np.array(ja, dtype=np.int32)
+        ii = np.array(ia, dtype=np.int32)
+        eigs = np.zeros(max(neig, 1))
+        res = np.zeros(max(neig, 1))
+        ic = np.array(icntl, dtype=np.int32)
+        ineig, iit, info = C.c_int(neig), C.c_int(maxiter), C.c_int(0)
+        sh, dt, mem, gap = C.c_double(0), C.c_double(1e-3), C.c_double(0), C.c_double(0)
+        L.jdo_pjd(s, n, _p(aa, C.c_double), _p(jj, C.c_int), _p(ii, C.c_int), _p(eigs, C.c_double),
+                  _p(res, C.c_double), _p(x, C.c_double), lx, C.byref(ineig), sigma, isearch,
+                  ninit, madspace, C.byref(iit), tol, C.byref(sh), C.byref(dt), C.byref(mem),
+                  _p(ic, C.c_int), 0, C.byref(info), C.byref(gap))
+        k = ineig.value if info.value >= 0 else 0
+        return Result(eigs=eigs[:k].copy(), res=res[:k].copy(),
+                      x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=iit.value,
+                      info=info.value, gap=gap.value, inner=L.jdo_inner_iters(s))
+    finally:
+        L.jdo_destroy(s)
+
+
+def ildl_apply(factor, r):
+    """z = M^-1 r for an exported multilevel factor (dict from
+    jadamilu_amd.ildl.IldlFactor.export()), sequential C restatement of the
+    multilevel substitution (src/ilupack/symAMGsol.c)."""
+    L = lib()
+    levs = factor["levels"]
+    arr = (_Level * max(len(levs), 1))()
+    keep = []
+    for k, lv in enumerate(levs):
+        fields = {}
+        for name in ("perm", "lb_ptr", "lb_idx", "lc_ptr", "lc_idx"):
+            a = np.ascontiguousarray(lv[name], dtype=np.int32)
+            keep.append(a)
+            fields[name] = a.ctypes.data_as(_i)
+        for name in ("scal", "lb_val", "lc_val", "d"):
+            a = np.ascontiguousarray(lv[name], dtype=np.float64)
+            if a.size == 0:
+                a = np.zeros(1)
+            keep.append(a)
+            fields[name] = a.ctypes.data_as(_d)
+        arr[k] = _Level(lv["n"], lv["nB"], **fields)
+    minv = np.ascontiguousarray(factor["minv"], dtype=np.float64).ravel()
+    if minv.size == 0:
+        minv = np.zeros(1)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    L.jdo_ildl_apply(len(levs), arr, factor["m"], minv.ctypes.data_as(_d), _p(r, C.c_double),
+                     _p(z, C.c_double))
+    return z
 
 
 def syev(m):

@@ -1,0 +1,87 @@
+"""CPU checks of the host multilevel ILDL factorisation.
+
+What pins the factor (no reference build is available):
+* exactness: with droptol = 0 nothing is dropped, so the multilevel factor
+  (sparse levels + dense tail) is an exact factorisation and M^-1 A x = x to
+  rounding, on tridiagonal, 2D-Laplace and indefinite Anderson matrices, with
+  several sparse levels forced (small dense switch);
+* inertia (Sylvester): with droptol = 0 the number of negative pivots equals
+  the number of eigenvalues of A - shift below zero (dense LAPACK count);
+* FP32 factor (the mixed path): exact factor to FP32 rounding;
+* the oracle apply (oracle/jdo_ildl.c) is the checker: a planted sign or
+  index error in it breaks M^-1 A x = x.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from jadamilu_amd import matrices as M
+from oracle import oracle as O
+
+ildl = pytest.importorskip("jadamilu_amd.ildl")
+
+
+def _relerr(ia, ja, a, shift, **kw):
+    n = ia.size - 1
+    f = ildl.IldlFactor(n, ia, ja, a, shift=shift, **kw)
+    assert f.ierr == 0
+    ex = f.export()
+    S = M.to_scipy(ia, ja, a) - shift * sp.eye(n)
+    x = np.random.default_rng(1).standard_normal(n)
+    z = O.ildl_apply(ex, S @ x)
+    return np.linalg.norm(z - x) / np.linalg.norm(x), f, ex
+
+
+CASES = [
+    ("tridiag", lambda: M.tridiag(300, np.arange(300) * 1.0, 5.0), 0.0),
+    ("laplace2d", lambda: M.laplace((14, 13)), 0.0),
+    ("anderson6", lambda: M.anderson(6, 16.5, 1), 0.0),
+    ("anderson7_shift", lambda: M.anderson(7, 12.0, 2), 0.37),
+]
+
+
+@pytest.mark.parametrize("name,gen,shift", CASES)
+@pytest.mark.parametrize("dense_max", [20, 1500])
+def test_exact_factor(name, gen, shift, dense_max):
+    ia, ja, a = gen()
+    err, f, ex = _relerr(ia, ja, a, shift, droptol=0.0, dense_max=dense_max)
+    assert err < 1e-11, err
+    if dense_max == 20 and name != "tridiag":
+        assert len(ex["levels"]) >= 1
+
+
+@pytest.mark.parametrize("name,gen,shift", CASES)
+def test_inertia(name, gen, shift):
+    ia, ja, a = gen()
+    n = ia.size - 1
+    w = np.linalg.eigvalsh(M.to_dense(ia, ja, a) - shift * np.eye(n))
+    f = ildl.IldlFactor(n, ia, ja, a, shift=shift, droptol=0.0, dense_max=20)
+    assert f.stats()["nneg"] == int((w < 0).sum())
+
+
+def test_single_precision_factor():
+    ia, ja, a = M.anderson(6, 16.5, 1)
+    err, f, ex = _relerr(ia, ja, a, 0.0, droptol=0.0, dense_max=20, single=True)
+    assert err < 5e-3, err
+    errd, _, _ = _relerr(ia, ja, a, 0.0, droptol=0.0, dense_max=20, single=False)
+    assert errd < err  # FP32 arithmetic visibly coarser
+
+
+def test_spd_conversion_positive():
+    """spd=True replaces pivots by |d|: the preconditioner is SPD."""
+    ia, ja, a = M.anderson(6, 16.5, 1)
+    n = ia.size - 1
+    f = ildl.IldlFactor(n, ia, ja, a, shift=0.0, droptol=1e-3, spd=True, dense_max=20)
+    ex = f.export()
+    P = np.column_stack([O.ildl_apply(ex, e) for e in np.eye(n)])
+    np.testing.assert_allclose(P, P.T, atol=1e-10 * np.abs(P).max())
+    assert np.linalg.eigvalsh(0.5 * (P + P.T)).min() > 0
+
+
+def test_droptol_reduces_fill_and_stays_useful():
+    ia, ja, a = M.laplace((24, 24, 24))
+    err, f, ex = _relerr(ia, ja, a, 0.0, droptol=1e-3, spd=True)
+    st = f.stats()
+    assert st["fill"] < 20 and err < 0.2
+    err0, f0, _ = _relerr(ia, ja, a, 0.0, droptol=1e-6, spd=True)
+    assert f0.stats()["fill"] > st["fill"] and err0 < err
