@@ -93,6 +93,14 @@ void jdhip_jdrevcom(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_
 /* As DPJDCLEANUP (PJD.F:435): release the preconditioner of ctx. */
 void jdhip_cleanup(jdhip_ctx *ctx);
 
+/* Reset ctx to the state of a fresh process: the preconditioner is released
+ * and the COMMON-block state of the reference (/PJDINITPJD/, /JDRVCOMJD/,
+ * JDaux.F:30-47, which persists between calls) is zeroed. */
+int jdhip_reset(jdhip_ctx *ctx);
+/* The process-global context behind the legacy symbols (created on first
+ * use on device $JDHIP_DEVICE, default 0). */
+jdhip_ctx *jdhip_legacy_context(void);
+
 /* HIP stream (as void*) the context launches on; callers servicing IJOB
  * with their own kernels should use it (or synchronise with it). */
 void *jdhip_stream(jdhip_ctx *ctx);
@@ -145,6 +153,19 @@ int jdhip_ildl_stats(jdhip_ildl *f, double *fill, int *nneg, double *condest, do
 int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps);
 /* dependency levels of the uploaded factor and algorithmic bytes per apply */
 int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels, long *apply_bytes);
+
+/* ---- eigenvalue adjoints (no reference counterpart) ---------------------- */
+/* Gradient of L = sum_k w[k] lambda_k (simple, gap-separated eigenvalues,
+ * unit eigenvectors X(:,k) = x_dev + k*ldx on the device) with respect to the
+ * stored entries of the 1-based upper CSR (ia_dev, ja_dev on the device):
+ * g[e(i,j)] = (i==j ? 1 : 2) * sum_k w[k] x_k(i) x_k(j).  w: host, k <= 64.
+ * Launched on `stream` (a hipStream_t, e.g. jdhip_stream(ctx)); 0 on success. */
+int jdhip_adjoint_csr(void *stream, int n, const int *ia_dev, const int *ja_dev,
+                      const double *x_dev, long ldx, int k, const double *w, double *g_dev);
+/* Gradient with respect to a diagonal potential V (A = H + diag(V)):
+ * g[i] = sum_k w[k] x_k(i)^2. */
+int jdhip_adjoint_diag(void *stream, int n, const double *x_dev, long ldx, int k,
+                       const double *w, double *g_dev);
 
 /* ---- diagnostics --------------------------------------------------------- */
 typedef struct {

@@ -57,6 +57,8 @@ SIGNATURES = {
     "jdhip_jdrevcom": (None, [_v, C.c_int, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
                               C.c_int, C.c_int, _i, C.c_double, _i, _l, _l, C.c_int, _i, _d]),
     "jdhip_cleanup": (None, [_v]),
+    "jdhip_reset": (C.c_int, [_v]),
+    "jdhip_legacy_context": (_v, []),
     "jdhip_stream": (_v, [_v]),
     "jdhip_set_matrix": (C.c_int, [_v, C.c_int, _i, _i, _d, C.c_int]),
     "jdhip_spmv": (C.c_int, [_v, _v, _v]),
@@ -73,6 +75,8 @@ SIGNATURES = {
     "jdhip_ildl_stats": (C.c_int, [_v, _d, _i, _d, _d]),
     "jdhip_ildl_attach": (C.c_int, [_v, _v, C.c_int]),
     "jdhip_ildl_device_info": (C.c_int, [_v, _i, _i, _l]),
+    "jdhip_adjoint_csr": (C.c_int, [_v, C.c_int, _v, _v, _v, C.c_long, C.c_int, _d, _v]),
+    "jdhip_adjoint_diag": (C.c_int, [_v, C.c_int, _v, C.c_long, C.c_int, _d, _v]),
     "jdhip_last_error": (C.c_char_p, [_v]),
 }
 

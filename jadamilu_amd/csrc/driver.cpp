@@ -37,6 +37,10 @@ struct SolverOps : Ops {
     void matvec(const double *x, double *y, const double *eta_dev, double eta, const double *u,
                 double *udot) override;
     void precond(JdEngine &e, double *z, const double *r) override;
+    bool f32 = false;
+    bool has_f32() const override { return f32; }
+    void matvec_f(const float *x, float *y, double eta, const float *u, double *udot) override;
+    void precond_f(JdEngine &e, float *z, const float *r) override;
 };
 
 struct Solver {
@@ -94,12 +98,15 @@ struct Solver {
         if (A.rowptr) hipFree(A.rowptr);
         if (A.col) hipFree(A.col);
         if (A.val) hipFree(A.val);
+        if (A.valf) hipFree(A.valf);
         A = CsrDev{};
         A_n = 0;
     }
     void release_precond() {
         if (d_diag) hipFree(d_diag);
+        if (d_diagf) hipFree(d_diagf);
         d_diag = nullptr;
+        d_diagf = nullptr;
         ndiag = 0;
         ildl.reset();
     }
@@ -135,6 +142,10 @@ struct Solver {
     }
     void dglprecsetup(int isearch_, double shift_in, double *shift0_out, double *diagmin_out);
     void precond_request(double *z, const double *r);
+    void precond_request_f(float *z, const float *r);
+    void adapt();
+    float *d_diagf = nullptr;
+    bool inner_fp32 = true;  // mixed path: FP32 inner solver when ICNTL(5) != 0
 
     void pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, double *res,
                   double *x_dev, long lx, int *neig_, double sigma_, int isearch_, int ninit,
@@ -145,9 +156,14 @@ struct Solver {
 
 void SolverOps::matvec(const double *x, double *y, const double *eta_dev, double eta,
                        const double *u, double *udot) {
-    spmv(S->ctx.stream, S->ctx.rb, S->A, x, y, eta_dev, eta, u, udot);
+    (void)eta_dev;
+    spmv<double>(S->ctx.stream, S->ctx.rb, S->A, x, y, eta, u, udot);
 }
 void SolverOps::precond(JdEngine &, double *z, const double *r) { S->precond_request(z, r); }
+void SolverOps::matvec_f(const float *x, float *y, double eta, const float *u, double *udot) {
+    spmv<float>(S->ctx.stream, S->ctx.rb, S->A, x, y, eta, u, udot);
+}
+void SolverOps::precond_f(JdEngine &, float *z, const float *r) { S->precond_request_f(z, r); }
 
 // Full (both triangles) CSR of the symmetric matrix, built on the host from
 // the upper triangle and uploaded once: the SpMV then reads each row with
@@ -187,6 +203,12 @@ void Solver::upload_matrix(int n_, const int *ia_, const int *ja_, const double 
     JDHIP_CHECK(hipMemcpy(A.rowptr, cnt.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice));
     JDHIP_CHECK(hipMemcpy(A.col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
     JDHIP_CHECK(hipMemcpy(A.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    {
+        // FP32 copy of the values for the mixed-precision inner solver
+        std::vector<float> vf(val.begin(), val.end());
+        JDHIP_CHECK(hipMalloc(&A.valf, sizeof(float) * std::max(nnz, 1L)));
+        JDHIP_CHECK(hipMemcpy(A.valf, vf.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+    }
     A.n = n_;
     A.nnz = nnz;
     A_n = n_;
@@ -208,9 +230,9 @@ int Solver::autotune_lpr() {
     float bt = 1e30f;
     for (int lpr : {1, 2, 4, 8, 16, 32}) {
         A.lpr = lpr;
-        spmv(ctx.stream, ctx.rb, A, x, y, nullptr, 0.0, nullptr, nullptr);
+        spmv<double>(ctx.stream, ctx.rb, A, x, y, 0.0, nullptr, nullptr);
         JDHIP_CHECK(hipEventRecord(e0, ctx.stream));
-        for (int r = 0; r < 10; ++r) spmv(ctx.stream, ctx.rb, A, x, y, nullptr, 0.0, nullptr, nullptr);
+        for (int r = 0; r < 10; ++r) spmv<double>(ctx.stream, ctx.rb, A, x, y, 0.0, nullptr, nullptr);
         JDHIP_CHECK(hipEventRecord(e1, ctx.stream));
         JDHIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
@@ -265,6 +287,9 @@ void Solver::dglprecsetup(int isearch_, double shift_in, double *shift0_out,
     JDHIP_CHECK(hipMemcpyAsync(d_diag, diag.data(), sizeof(double) * n, hipMemcpyHostToDevice,
                                ctx.stream));
     JDHIP_CHECK(hipStreamSynchronize(ctx.stream));
+    if (d_diagf) JDHIP_CHECK(hipFree(d_diagf));
+    JDHIP_CHECK(hipMalloc(&d_diagf, sizeof(float) * n));
+    convert(ctx.stream, d_diagf, d_diag, n);
     factdgl = 1;
 }
 
@@ -412,8 +437,23 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
     ierr = -54321;
 }
 
-// IJOB=2 branch of PJDRVCOM (JDaux.F:282-471)
 void Solver::precond_request(double *z, const double *r) {
+    adapt();
+    if (factdgl > 0) diag_apply<double>(ctx.stream, z, d_diag, r, n);
+    else if (ildl) ildl->apply(z, r);
+    else throw std::runtime_error("no preconditioner");
+}
+
+void Solver::precond_request_f(float *z, const float *r) {
+    adapt();
+    if (factdgl > 0) diag_apply<float>(ctx.stream, z, d_diagf, r, n);
+    else if (ildl) ildl->impl()->apply_f(z, r);
+    else throw std::runtime_error("no preconditioner");
+}
+
+// IJOB=2 branch of PJDRVCOM (JDaux.F:282-449): decide whether the
+// preconditioner is rebuilt before this solve.
+void Solver::adapt() {
     bool reinit = false;
     int factspd0 = 0;
     double tent = 0.0;
@@ -517,13 +557,6 @@ void Solver::precond_request(double *z, const double *r) {
         tic_ = std::chrono::steady_clock::now();
         if (factspd0 < 0 && factspd <= 0) factspd = factspd0;
     }
-    if (factdgl > 0) {
-        diag_apply(ctx.stream, z, d_diag, r, n);
-    } else if (ildl) {
-        ildl->apply(z, r);
-    } else {
-        throw std::runtime_error("no preconditioner");
-    }
 }
 
 void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, double *res,
@@ -546,7 +579,6 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
     tol_ = tol;
     gap_ = gap ? gap : &gap_dummy_;
     if (*ijob == 0) {
-        flagsingle = icntl[4] != 0;
         ierr = 0;
         shifts_log.clear();
         nsetups = 0;
@@ -554,13 +586,31 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
         if ((icntl[1] != -1 && icntl[1] != -2) || factdgl == 0) {
             prvdr = 0;
             timefact = -1.0;
+            // ICNTL(5) selects the factor precision at factorisation time; a
+            // reused factor keeps the precision it was built with
+            flagsingle = icntl[4] != 0;
             pjdinit(isearch, sigma, *shift);
             reenterfirsttime = false;
         } else {
+            // reuse of the existing preconditioner (JDaux.F:146-244): only
+            // the shift bookkeeping is refreshed, the factor is kept
             prvdr = 0;
             reenterfirsttime = true;
             eng.rinit = eng.rcurr = eng.rcum = 0.0;
-            if (ja[0] < 0 && isearch >= 2) shift0 = sigma;
+            rprev = 0.0;
+            if (ja[0] < 0) {
+                if (isearch >= 2) shift0 = sigma;
+            } else {
+                double dmin = 0.0;
+                for (int i = 0; i < n; ++i)
+                    for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+                        if (ja[k] == i + 1) dmin = (i == 0) ? a[k] : std::min(dmin, a[k]);
+                diagmin = dmin;
+                shift0 = savediag_shift();
+                if (isearch >= 2) shift0 = sigma;
+                if (isearch == 1 && *shift < diagmin) shift0 = *shift;
+                eng.timeilu = 0.0;
+            }
         }
         if (ierr != 0) {
             *info = -54321;
@@ -572,6 +622,9 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
         tic_ = std::chrono::steady_clock::now();
         ops.mv = internal_mv;
         ops.pc = true;
+        // mixed path (ICNTL(5) != 0): FP32 factor and FP32 inner solver;
+        // the outer residual, orthogonalisation and Rayleigh-Ritz stay FP64
+        ops.f32 = internal_mv && flagsingle && inner_fp32;
         eng.set_ops(&ops);
     }
     eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
@@ -684,7 +737,7 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev) {
     try {
         Solver &S = *ctx->S;
         if (!S.A.rowptr) throw std::runtime_error("jdhip_spmv: no matrix set");
-        jdhip::spmv(S.ctx.stream, S.ctx.rb, S.A, x_dev, y_dev, nullptr, 0.0, nullptr, nullptr);
+        jdhip::spmv<double>(S.ctx.stream, S.ctx.rb, S.A, x_dev, y_dev, 0.0, nullptr, nullptr);
         return 0;
     } catch (const std::exception &e) {
         set_err(ctx, e.what());
@@ -773,6 +826,21 @@ void jdhip_cleanup(jdhip_ctx *ctx) {
     ctx->S->factdgl = 0;
 }
 
+int jdhip_reset(jdhip_ctx *ctx) {
+    if (!ctx) return -1;
+    Solver &S = *ctx->S;
+    S.release_precond();
+    S.timefact = S.shift0 = S.diagmin = S.shiftmax = 0.0;
+    S.memrequested = S.memused = S.droptol0 = S.condest0 = 0.0;
+    S.factdgl = S.factspd = S.prvdr = 0;
+    S.reenterfirsttime = S.flagsingle = false;
+    S.nw = S.ierr = 0;
+    S.rprev = 0.0;
+    S.eng.timeilu = S.eng.rinit = S.eng.rcurr = S.eng.rcum = S.eng.shift1 = 0.0;
+    S.eng.flagcg = false;
+    return 0;
+}
+
 int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     if (!ctx || !name) return -1;
     Solver &S = *ctx->S;
@@ -781,6 +849,7 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_ordering") S.ildl_ordering = (int)value;
     else if (k == "ildl_dense_max") S.ildl_dense_max = (int)value;
     else if (k == "spmv_lanes") S.A.lpr = (int)value;
+    else if (k == "inner_fp32") S.inner_fp32 = value != 0.0;
     else return -2;
     return 0;
 }
@@ -1058,6 +1127,8 @@ void dpjdcleanup_(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_ctx) jdhip_cleanup(g_ctx);
 }
+
+jdhip_ctx *jdhip_legacy_context(void) { return legacy_ctx(); }
 
 void dpjd(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x, int *lx,
           int *neig, double *sigma, int *isearch, int *ninit, int *madspace, int *iter,

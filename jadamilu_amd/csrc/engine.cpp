@@ -81,6 +81,17 @@ void JdEngine::put(double *dev, const double *host, long cnt) {
     ctx_->sync();
 }
 
+void JdEngine::ensure_f32(long n, int nq) {
+    long need = n * (6L + nq);
+    if (need <= fbuf_len_) return;
+    if (fbuf_) {
+        ctx_->sync();
+        JDHIP_CHECK(hipFree(fbuf_));
+    }
+    JDHIP_CHECK(hipMalloc(&fbuf_, sizeof(float) * need));
+    fbuf_len_ = need;
+}
+
 double JdEngine::dnorm(const double *w, long n) {
     double *d = ctx_->slot(2);
     dot(ctx_->stream, ctx_->rb, w, w, n, d);
@@ -656,7 +667,18 @@ L20:  // expansion
         ifo = 0;
         ls_maxit = itcg;
         ls_iter = 0;
-        if (ls_posdef) fill(s, D(xcg), 0.0, n);
+        f32_ = use_f32();
+        if (f32_) {
+            // mixed path: the correction equation is solved on FP32 copies
+            // of r and of the projector basis [u, X]
+            ensure_f32(n, nwant + 1);
+            convert(s, fbuf_, D(wcg), n);
+            convert(s, fbuf_ + 6L * n, D(uu), (long)(*neig + 1) * n);
+            if (ls_posdef)
+                JDHIP_CHECK(hipMemsetAsync(fbuf_ + 4L * n, 0, sizeof(float) * n, s));
+        } else if (ls_posdef) {
+            fill(s, D(xcg), 0.0, n);
+        }
         InnerScalars &h = *ctx_->h_sc;
         h.posdef = ls_posdef ? 1 : 0;
         h.flag = 0;
@@ -669,6 +691,23 @@ L20:  // expansion
 L10i:  // new inner iteration: preconditioner solve Z = P^-1 R
     ++ls_iter;
     ++inner_iters;
+    if (f32_) {
+        float *rf = fbuf_, *zf = fbuf_ + n, *pf = fbuf_ + 2L * n, *qf = fbuf_ + 3L * n;
+        float *xf = fbuf_ + 4L * n, *x2f = fbuf_ + 5L * n, *Qf = fbuf_ + 6L * n;
+        ops_->precond_f(*this, zf, rf);
+        flagcg = true;
+        cg_project_rho<float>(s, rb, zf, Qf, *neig + 1, rf, n, ctx_->d_sc, ctx_->slot(0));
+        cg_direction<float>(s, zf, pf, n, ctx_->d_sc, ls_iter);
+        ops_->matvec_f(pf, qf, eta, Qf, &ctx_->d_sc->alprho);
+        cg_project_alpha<float>(s, rb, qf, Qf, pf, n, ctx_->d_sc);
+        if (ls_posdef) {
+            cg_update_posdef<float>(s, rb, rf, qf, xf, pf, n, ctx_->d_sc);
+        } else {
+            qmr_update_r<float>(s, rb, rf, qf, n, ctx_->d_sc);
+            qmr_update_x<float>(s, rb, xf, x2f, pf, n, ctx_->d_sc, ls_iter);
+        }
+        goto Lsync;
+    }
     *ndx1 = wcg + n;  // Z
     *ndx2 = wcg;      // R
     rlbl = 4;
@@ -712,6 +751,7 @@ L4i:
         qmr_update_r(s, rb, D(wcg), D(wcg + 3L * n), n, ctx_->d_sc);
         qmr_update_x(s, rb, D(xcg), D(xcg2), D(wcg + 2L * n), n, ctx_->d_sc, ls_iter);
     }
+Lsync:
     JDHIP_CHECK(hipMemcpyAsync(ctx_->h_sc, ctx_->d_sc, sizeof(InnerScalars),
                                hipMemcpyDeviceToHost, s));
     ctx_->sync();
@@ -745,9 +785,19 @@ L4i:
 L40i:  // inner iteration fails (JDaux.F:3255-3266)
     ifo = 1;
     if (alpha <= 0.0) ifo = 2;
-    if (ls_iter == 1)
-        JDHIP_CHECK(hipMemcpyAsync(D(xcg), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
+    if (ls_iter == 1) {
+        if (f32_)
+            JDHIP_CHECK(hipMemcpyAsync(fbuf_ + 4L * n, fbuf_ + 2L * n, sizeof(float) * n,
+                                       hipMemcpyDeviceToDevice, s));
+        else
+            JDHIP_CHECK(
+                hipMemcpyAsync(D(xcg), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
+    }
 L34:
+    if (f32_) {
+        convert(s, D(xcg), fbuf_ + 4L * n, n);  // correction back to FP64
+        f32_ = false;
+    }
     itcg = ls_iter;
     if (iserch != 2 && eta <= eta0 && *neig > 0 && ifo == 2) eta0 = std::min(eta1, eta0);
     conv = false;

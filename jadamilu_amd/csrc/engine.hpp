@@ -63,11 +63,19 @@ struct Ops {
     virtual bool has_precond() const { return false; }
     // called when the reference would return IJOB=2 to PJDRVCOM
     virtual void precond(JdEngine &e, double *z, const double *r) {}
+    // FP32 inner solver (mixed path): operator and preconditioner on float
+    // vectors; only used when has_f32() and both operators are inline.
+    virtual bool has_f32() const { return false; }
+    virtual void matvec_f(const float *x, float *y, double eta, const float *u, double *udot) {}
+    virtual void precond_f(JdEngine &e, float *z, const float *r) {}
 };
 
 class JdEngine {
   public:
     explicit JdEngine(DevCtx *ctx) : ctx_(ctx) {}
+    ~JdEngine() {
+        if (fbuf_) (void)hipFree(fbuf_);
+    }
 
     // One JDRVCOM call.  x is the DEVICE workspace (lx doubles), eigs/res
     // host arrays.  On return *ijob: 0 done, 1 matvec X(ndx2)=A X(ndx1),
@@ -100,6 +108,12 @@ class JdEngine {
                   double eta, int *info, double trs, double xn2);
     bool internal_mv() const { return ops_ && ops_->has_matvec(); }
     bool internal_pc() const { return ops_ && ops_->has_precond(); }
+    bool use_f32() const { return internal_mv() && internal_pc() && ops_->has_f32(); }
+    void ensure_f32(long n, int nq);
+    // FP32 inner-solver buffers: r z p q x x2 (n each) and Q ((NEIG+1) n)
+    float *fbuf_ = nullptr;
+    long fbuf_len_ = 0;
+    bool f32_ = false;
     void do_matvec(const double *xin, double *yout) {
         ops_->matvec(xin, yout, nullptr, 0.0, nullptr, nullptr);
     }

@@ -186,19 +186,23 @@ class IldlImpl : public IldlImplBase {
         JDHIP_CHECK(hipMemcpyAsync(D.buf, xin, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
     }
 
-    void apply(double *z, const double *r) override {
+    void apply(double *z, const double *r) override { apply_io<double>(z, r); }
+    void apply_f(float *z, const float *r) override { apply_io<float>(z, r); }
+
+    template <typename TV>
+    void apply_io(TV *z, const TV *r) {
         hipStream_t s = ctx_->stream;
         const int L = (int)lv_.size();
         if (L == 0) {
             // only a dense level: z = Minv r
-            ildl_gather<double, T>(s, f_.m, ident_(), ones_(), r, dx_);
+            ildl_gather<TV, T>(s, f_.m, ident_(), ones_(), r, dx_);
             ildl_dense_gemv<T>(s, f_.m, dmat_, dx_, dy_);
-            ildl_scatter<T, double>(s, f_.m, f_.m, ident_(), ones_(), dy_, dy_, z);
+            ildl_scatter<T, TV>(s, f_.m, f_.m, ident_(), ones_(), dy_, dy_, z);
             return;
         }
         for (int l = 0; l < L; ++l) {
             DevLevel<T> &D = lv_[l];
-            if (l == 0) ildl_gather<double, T>(s, D.n, D.perm, D.scal, r, D.buf);
+            if (l == 0) ildl_gather<TV, T>(s, D.n, D.perm, D.scal, r, D.buf);
             else {
                 const DevLevel<T> &P = lv_[l - 1];
                 ildl_gather<T, T>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
@@ -219,7 +223,7 @@ class IldlImpl : public IldlImplBase {
             if (D.nC > 0)
                 ildl_spmv_sub<T>(s, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, zc, D.buf);
             tri_solve(D, false);
-            if (l == 0) ildl_scatter<T, double>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
+            if (l == 0) ildl_scatter<T, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
             else ildl_scatter<T, T>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
             zc = D.out;
         }

@@ -35,6 +35,7 @@ class IldlImplBase {
   public:
     virtual ~IldlImplBase() = default;
     virtual void apply(double *z, const double *r) = 0;
+    virtual void apply_f(float *z, const float *r) = 0;  // FP32 in/out
     virtual int nlevels() const = 0;
     virtual int dense_m() const = 0;
     // level data in double precision (host copies of the factor)

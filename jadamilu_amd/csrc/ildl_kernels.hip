@@ -167,6 +167,10 @@ template void ildl_gather<double, float>(hipStream_t, int, const int *, const fl
                                          const double *, float *);
 template void ildl_gather<float, float>(hipStream_t, int, const int *, const float *,
                                         const float *, float *);
+template void ildl_gather<float, double>(hipStream_t, int, const int *, const double *,
+                                         const float *, double *);
+template void ildl_scatter<double, float>(hipStream_t, int, int, const int *, const double *,
+                                          const double *, const double *, float *);
 template void ildl_scatter<double, double>(hipStream_t, int, int, const int *, const double *,
                                            const double *, const double *, double *);
 template void ildl_scatter<float, double>(hipStream_t, int, int, const int *, const float *,

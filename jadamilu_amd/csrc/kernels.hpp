@@ -38,9 +38,10 @@ int red_blocks(long n);
 void fill(hipStream_t s, double *x, double v, long n);
 void scal(hipStream_t s, double *x, double a, long n);
 void axpy(hipStream_t s, double *y, double a, const double *x, long n);
-// out[c] = sum_i V[c*ldv+i] * w[i], c < jv  (result in rb.out, device)
-void multidot(hipStream_t s, const RedBuf &rb, const double *V, long ldv, int jv,
-              const double *w, long n, double *out);
+// out[c] = sum_i V[c*ldv+i] * w[i], c < jv (device, accumulated in double)
+template <typename T>
+void multidot(hipStream_t s, const RedBuf &rb, const T *V, long ldv, int jv, const T *w, long n,
+              double *out);
 // w = beta*w + alpha * V * y (y device, jv entries); if nrm2out, also
 // nrm2out[0] = ||w||^2 after the update.
 void gemv_n(hipStream_t s, const RedBuf &rb, double *w, const double *V, long ldv,
@@ -68,30 +69,39 @@ struct CsrDev {
     float *valf = nullptr;  // optional FP32 copy
     int lpr = 8;            // lanes per row
 };
-// y = A x ; if eta_ptr: y -= (*eta_ptr) x ; if u: out[0] = u.y (after)
-void spmv(hipStream_t s, const RedBuf &rb, const CsrDev &A, const double *x, double *y,
-          const double *eta_dev, double eta_host, const double *u, double *udot_out);
+// y = A x (T = double: A.val, T = float: A.valf); if eta != 0: y -= eta x;
+// if u: *udot_out = u.y (double)
+template <typename T>
+void spmv(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T *x, T *y, double eta,
+          const T *u, double *udot_out);
 
-// ---- preconditioners ----
-void diag_apply(hipStream_t s, double *z, const double *d, const double *r, long n);
+// ---- preconditioners / conversion ----
+template <typename T>
+void diag_apply(hipStream_t s, T *z, const T *d, const T *r, long n);
+void convert(hipStream_t s, float *dst, const double *src, long n);
+void convert(hipStream_t s, double *dst, const float *src, long n);
 
-// ---- inner solver (LSOLREVCOM) fused steps ----
-// z = P^-1 r is already in z.  y = Q^T z (jq columns) -> rb.out region yout
-// then z -= Q y ; rho = r.z ; checks ; p = z (+ beta p)
-void cg_project_rho(hipStream_t s, const RedBuf &rb, double *z, const double *Q, int jq,
-                    const double *r, long n, InnerScalars *sc, double *ycoef);
-void cg_direction(hipStream_t s, double *z, double *p, long n, InnerScalars *sc, int iter);
-// q already = (A - eta) p ; alprho = u.q (device, computed by spmv) ;
-// q -= u alprho ; pq = p.q ; alpha = rho/pq with checks
-void cg_project_alpha(hipStream_t s, const RedBuf &rb, double *q, const double *u,
-                      const double *p, long n, InnerScalars *sc);
+// ---- inner solver (LSOLREVCOM) fused steps, T = double or float ----
+// z = P^-1 r is already in z: ycoef = Q^T z ; z -= Q ycoef ; rho = r.z
+template <typename T>
+void cg_project_rho(hipStream_t s, const RedBuf &rb, T *z, const T *Q, int jq, const T *r,
+                    long n, InnerScalars *sc, double *ycoef);
+// z += (rho/rho1) p ; p = z   (iteration number from the host)
+template <typename T>
+void cg_direction(hipStream_t s, T *z, T *p, long n, InnerScalars *sc, int iter);
+// q already = (A - eta) p, alprho = u.q ; q -= u alprho ; pq = p.q ; alpha
+template <typename T>
+void cg_project_alpha(hipStream_t s, const RedBuf &rb, T *q, const T *u, const T *p, long n,
+                      InnerScalars *sc);
 // PCG: r -= alpha q ; gk2 = r.r ; x += alpha p ; xn2 = x.x
-void cg_update_posdef(hipStream_t s, const RedBuf &rb, double *r, const double *q,
-                      double *x, const double *p, long n, InnerScalars *sc);
-// sQMR: r -= alpha q ; gk2 = r.r, then the smoothing update of x, x2
-void qmr_update_r(hipStream_t s, const RedBuf &rb, double *r, const double *q, long n,
-                  InnerScalars *sc);
-void qmr_update_x(hipStream_t s, const RedBuf &rb, double *x, double *x2, const double *p,
-                  long n, InnerScalars *sc, int iter);
+template <typename T>
+void cg_update_posdef(hipStream_t s, const RedBuf &rb, T *r, const T *q, T *x, const T *p,
+                      long n, InnerScalars *sc);
+// sQMR: r -= alpha q ; gk2 ; smoothing scalars ; then x, x2 update
+template <typename T>
+void qmr_update_r(hipStream_t s, const RedBuf &rb, T *r, const T *q, long n, InnerScalars *sc);
+template <typename T>
+void qmr_update_x(hipStream_t s, const RedBuf &rb, T *x, T *x2, const T *p, long n,
+                  InnerScalars *sc, int iter);
 
 }  // namespace jdhip

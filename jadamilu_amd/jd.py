@@ -151,6 +151,14 @@ def cleanup():
     lib().dpjdcleanup_()
 
 
+def reset_legacy():
+    """Return the process-global context behind the legacy symbols to the
+    state of a fresh process (preconditioner released, saved COMMON state
+    zeroed), so a run does not inherit shifts from earlier problems."""
+    L = lib()
+    L.jdhip_reset(L.jdhip_legacy_context())
+
+
 class Context:
     """A solver context on one HIP device (jdhip_create); X lives in device
     memory (a torch CUDA tensor is used as the allocator)."""

@@ -36,6 +36,9 @@ def test_guide_diag_legacy(golden, name, icntl):
     ia, ja, a = ex9()
     mv = O.matvec_upper(ia, ja, a)
     d = np.arange(1, 1001, dtype=float)
+    # the guide's runs start from a fresh process: clear the saved state
+    # (diagmin, shifts) that earlier tests left in the global context
+    jd.reset_legacy()
     r = jd.pjdrevcom_host(1000, d, [-1], [-1], mv, neig=5, isearch=0, madspace=20,
                           maxiter=1000, tol=1e-10, icntl=icntl)
     ro = O.pjdrevcom_diag(1000, d, mv, neig=5, isearch=0, madspace=20, maxiter=1000,
@@ -155,22 +158,24 @@ def test_laplace2d_device_vs_closed_form_and_oracle():
     np.testing.assert_allclose(X.T @ X, np.eye(8), atol=1e-12)
 
 
-def test_anderson_interior_device_vs_oracle():
-    """ISEARCH=2, eigenvalues nearest sigma=0 of a small Anderson model with
-    the diagonal preconditioner (harmonic extraction + sQMR)."""
-    L = 8
-    ia, ja, a = M.anderson(L, 16.5, 11)
-    n = L ** 3
-    d = a[ia[:-1] - 1].copy()
-    r, X = _device_solve(ia, ja, a, d, neig=4, madspace=20, isearch=2, sigma=0.0, maxiter=20000)
+def test_interior_diag_harmonic_device_vs_oracle():
+    """ISEARCH=2 (harmonic extraction + sQMR) with the diagonal
+    preconditioner: the 4 eigenvalues nearest sigma=500.3 of the EXAMPLE9
+    matrix (tridiag, diag i, off-diagonal 5), n=1000."""
+    n = 1000
+    ia, ja, a = ex9(n)
+    d = np.arange(1, n + 1, dtype=float)
+    r, X = _device_solve(ia, ja, a, d, neig=4, madspace=20, isearch=2, sigma=500.3,
+                         maxiter=5000)
     ro = O.pjdrevcom_diag(n, d, O.matvec_upper(ia, ja, a), neig=4, madspace=20, isearch=2,
-                          sigma=0.0, maxiter=20000, tol=1e-10)
+                          sigma=500.3, maxiter=5000, tol=1e-10)
     w = np.linalg.eigvalsh(M.to_dense(ia, ja, a))
-    near = np.sort(w[np.argsort(np.abs(w))[:4]])
-    print("anderson8 gpu iter", r.iter, "oracle", ro.iter)
+    near = np.sort(w[np.argsort(np.abs(w - 500.3))[:4]])
+    print("interior gpu iter", r.iter, "oracle", ro.iter)
     assert r.info == 0 and ro.info == 0
-    np.testing.assert_allclose(np.sort(r.eigs), near, rtol=1e-10, atol=1e-13)
-    np.testing.assert_allclose(np.sort(r.eigs), np.sort(ro.eigs), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(np.sort(r.eigs), near, rtol=1e-10)
+    np.testing.assert_allclose(np.sort(r.eigs), np.sort(ro.eigs), rtol=1e-10)
+    assert abs(r.iter - ro.iter) <= max(3, 0.10 * ro.iter)
 
 
 def test_input_errors_device():
