@@ -93,6 +93,13 @@ void jdhip_jdrevcom(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_
 /* As DPJDCLEANUP (PJD.F:435): release the preconditioner of ctx. */
 void jdhip_cleanup(jdhip_ctx *ctx);
 
+/* Approximate minimum degree ordering of the symmetric pattern given by a
+ * 1-based upper (or lower) triangle CSR; perm[k] (1-based) is the original
+ * index placed at position k.  Replaces the AMD call that orders each level
+ * of the reference's factorization (src/ilupack/permamd.c,
+ * src/ilupack/sympermMC64amd.c:503-509).  Returns 0, or -1 on bad input. */
+int jdhip_order_amd(int n, const int *ia, const int *ja, int *perm);
+
 /* Reset ctx to the state of a fresh process: the preconditioner is released
  * and the COMMON-block state of the reference (/PJDINITPJD/, /JDRVCOMJD/,
  * JDaux.F:30-47, which persists between calls) is zeroed. */
@@ -117,7 +124,8 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
 /* Build options of ctx (no reference counterpart; defaults reproduce the
  * reference's behaviour): "ildl_sweeps" (0 = exact level-scheduled
  * triangular sweeps, k > 0 = k Jacobi sweeps per triangular solve),
- * "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee), "ildl_dense_max"
+ * "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum
+ * degree, the default), "ildl_dense_max"
  * (largest final level factored densely), "spmv_lanes".  0 on success. */
 int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value);
 

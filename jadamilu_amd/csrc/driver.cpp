@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "ordering.hpp"
 #include "../../include/jadamilu_amd.h"
 #include "engine.hpp"
 #include "ildl.hpp"
@@ -55,7 +56,7 @@ struct Solver {
     double slightlyless = 0.8, toldiv = 2.0;  // src/blaslike/common.f:14-15
     int factdgl = 0, factspd = 0, prvdr = 0;
     // build options of the multilevel ILDL
-    int ildl_sweeps = 0, ildl_ordering = 1, ildl_dense_max = 1500;
+    int ildl_sweeps = 0, ildl_ordering = 2, ildl_dense_max = 1500;
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -824,6 +825,15 @@ void jdhip_cleanup(jdhip_ctx *ctx) {
     if (!ctx) return;
     ctx->S->release_precond();
     ctx->S->factdgl = 0;
+}
+
+int jdhip_order_amd(int n, const int *ia, const int *ja, int *perm) {
+    if (n < 0 || !ia || !ja || !perm) return -1;
+    if (n == 0) return 0;
+    jdhip::SymCsr S = jdhip::sym_from_upper(n, ia, ja, nullptr, 0.0);
+    std::vector<int> ord = jdhip::amd_order(n, S.ptr.data(), S.idx.data());
+    for (int k = 0; k < n; ++k) perm[k] = ord[k] + 1;
+    return 0;
 }
 
 int jdhip_reset(jdhip_ctx *ctx) {

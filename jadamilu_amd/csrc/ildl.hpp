@@ -16,7 +16,7 @@ struct IldlOptions {
     bool spd = false;    // |D| conversion (ISEARCH <= 1)
     bool single = false; // FP32 factor, storage and apply (ICNTL(5) != 0)
     int sweeps = 0;      // 0: exact level-scheduled sweeps; k > 0: k Jacobi sweeps
-    int ordering = 1;
+    int ordering = 2;  // 0 natural, 1 RCM, 2 AMD
     int dense_max = 1500;
 };
 

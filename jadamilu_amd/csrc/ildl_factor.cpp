@@ -1,6 +1,7 @@
 // ildl_factor.cpp -- host multilevel inverse-based incomplete LDL^T (see
 // ildl_factor.hpp for the reference routines each step restates).
 #include "ildl_factor.hpp"
+#include "ordering.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -37,13 +38,14 @@ SymCsr sym_from_upper(int n, const int *ia, const int *ja, const double *a, doub
     for (int i = 0; i < n; ++i)
         for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
             int j = ja[k] - 1;
+            const double v = a ? a[k] : 0.0;  // a == nullptr: pattern only
             if (j == i) {
-                S.val[S.ptr[i]] += a[k];
+                S.val[S.ptr[i]] += v;
             } else {
                 S.idx[fill[i]] = j;
-                S.val[fill[i]++] = a[k];
+                S.val[fill[i]++] = v;
                 S.idx[fill[j]] = i;
-                S.val[fill[j]++] = a[k];
+                S.val[fill[j]++] = v;
             }
         }
     // sort each row by column, merging duplicates
@@ -174,7 +176,8 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     }
     // ---- ordering ----
     std::vector<int> order;
-    if (prm.ordering == 1) order = rcm_order(A);
+    if (prm.ordering == 2) order = amd_order(n, A.ptr.data(), A.idx.data());
+    else if (prm.ordering == 1) order = rcm_order(A);
     else {
         order.resize(n);
         std::iota(order.begin(), order.end(), 0);

@@ -56,7 +56,7 @@ struct IldlParams {
     int dense_max = 1500;    // final level solved densely when n <= dense_max
     int max_levels = 40;
     bool spd = false;        // convert pivots to |d| (ISEARCH <= 1)
-    int ordering = 1;        // 0 natural, 1 reverse Cuthill-McKee
+    int ordering = 2;        // 0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum degree
 };
 
 struct IldlStats {

@@ -22,7 +22,7 @@ def _p(a, t):
 
 class IldlFactor:
     def __init__(self, n, ia, ja, a, *, shift=0.0, droptol=1e-3, condest=5.0, spd=False,
-                 single=False, ordering=1, dense_max=1500):
+                 single=False, ordering=2, dense_max=1500):
         L = lib()
         self._L = L
         ia = np.ascontiguousarray(ia, dtype=np.int32)
