@@ -114,7 +114,10 @@ void *jdhip_stream(jdhip_ctx *ctx);
 
 /* ---- symmetric SpMV (MATVECA, JDaux.F:3397-3418) ------------------------- */
 /* Upload a 1-based upper-triangle CSR (diagonal anywhere in each row) as the
- * device operator of ctx; 0 on success.  lanes_per_row: 0 = autotune. */
+ * device operator of ctx; 0 on success.  lanes_per_row: 0 = autotune (the
+ * formats below timed on this matrix), -1 = sliced ELL (64-row slices, one
+ * lane per row, 16-bit column offsets where a slice allows), k > 0 = CSR with
+ * k lanes per row. */
 int jdhip_set_matrix(jdhip_ctx *ctx, int n, const int *ia, const int *ja, const double *a,
                      int lanes_per_row);
 /* y = A x on ctx's stream, x and y DEVICE arrays of n doubles. */
@@ -122,8 +125,10 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
 
 /* ---- options ------------------------------------------------------------- */
 /* Build options of ctx (no reference counterpart; defaults reproduce the
- * reference's behaviour): "ildl_sweeps" (0 = exact level-scheduled
- * triangular sweeps, k > 0 = k Jacobi sweeps per triangular solve),
+ * reference's behaviour, except "ildl_sweeps"): "ildl_sweeps" (0 = exact
+ * level-scheduled triangular solves, k > 0 = k Jacobi sweeps per triangular
+ * solve, -1 = the default: chosen at set-up by timing both on the device,
+ * sweeps only where they reproduce the exact apply to 3 %),
  * "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum
  * degree, the default), "ildl_dense_max"
  * (largest final level factored densely), "spmv_lanes".  0 on success. */
@@ -180,12 +185,13 @@ typedef struct {
     long inner_iters;     /* inner PCG/sQMR steps of the last call */
     long host_syncs;      /* host<->device synchronisations of the last call */
     int n_precond_setups; /* preconditioner (re)factorisations */
-    int spmv_lanes;       /* lanes per row chosen for the SpMV */
+    int spmv_lanes;       /* SpMV format chosen: k lanes per row, 0 sliced ELL */
     double setup_shifts[64];
     double precond_seconds; /* host factorisation + upload time of the last call */
     double fill;            /* stored factor entries / nnz(upper A) (ILDL) */
     int precond_levels;     /* sparse levels + dense level (ILDL) */
     int precond_negpiv;     /* negative pivots before |D| conversion */
+    int precond_sweeps;     /* triangular solves: 0 exact, k Jacobi sweeps */
 } jdhip_stats;
 void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out);
 /* last error message of ctx ("" if none) */

@@ -56,7 +56,7 @@ struct Solver {
     double slightlyless = 0.8, toldiv = 2.0;  // src/blaslike/common.f:14-15
     int factdgl = 0, factspd = 0, prvdr = 0;
     // build options of the multilevel ILDL
-    int ildl_sweeps = 0, ildl_ordering = 2, ildl_dense_max = 1500;
+    int ildl_sweeps = -1, ildl_ordering = 2, ildl_dense_max = 1500;
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -100,6 +100,9 @@ struct Solver {
         if (A.col) hipFree(A.col);
         if (A.val) hipFree(A.val);
         if (A.valf) hipFree(A.valf);
+        for (void *p : {(void *)A.s_vptr, (void *)A.s_iptr, (void *)A.s_wm, (void *)A.s_val,
+                        (void *)A.s_valf, (void *)A.s_idx})
+            if (p) hipFree(p);
         A = CsrDev{};
         A_n = 0;
     }
@@ -119,6 +122,8 @@ struct Solver {
     }
 
     void upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr);
+    void build_sell(int n_, const std::vector<int> &rp, const std::vector<int> &col,
+                    const std::vector<double> &val);
     double A_fp = 0.0;
     static double fingerprint(int n_, const int *ia_, const int *ja_, const double *a_) {
         double s = (double)n_ + 1e-3 * (double)ia_[n_];
@@ -213,8 +218,74 @@ void Solver::upload_matrix(int n_, const int *ia_, const int *ja_, const double 
     A.n = n_;
     A.nnz = nnz;
     A_n = n_;
+    build_sell(n_, cnt, col, val);
     A.lpr = lpr > 0 ? lpr : 8;
-    if (lpr <= 0) A.lpr = autotune_lpr();
+    if (lpr < 0 && A.nslices > 0) A.lpr = 0;
+    if (lpr == 0) A.lpr = autotune_lpr();
+}
+
+// Sliced-ELL copy of the full CSR: slices of 64 rows, entries of slice s at
+// element offsets vptr[s] + k*64 + lane (k < width of the slice); padding
+// holds value 0 at the row's own column.  Skipped when padding would more
+// than double the stored entries.
+void Solver::build_sell(int n_, const std::vector<int> &rp, const std::vector<int> &col,
+                        const std::vector<double> &val) {
+    const int ns = (n_ + 63) / 64;
+    std::vector<int> wm(ns);
+    std::vector<long> vptr(ns + 1, 0), iptr(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        int w = 0;
+        bool short_ok = true;
+        for (int r = s * 64; r < std::min(n_, s * 64 + 64); ++r) {
+            w = std::max(w, rp[r + 1] - rp[r]);
+            for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                const long d = (long)col[k] - r;
+                if (d < -32768 || d > 32767) short_ok = false;
+            }
+        }
+        wm[s] = 2 * w + (short_ok ? 1 : 0);
+        vptr[s + 1] = vptr[s] + 64L * w;
+        iptr[s + 1] = iptr[s] + 64L * w * (short_ok ? 2 : 4);
+    }
+    const long ent = vptr[ns];
+    if (ent > 2 * (long)rp[n_] + 64L * ns) return;
+    std::vector<double> sv(ent, 0.0);
+    std::vector<char> sidx(std::max(iptr[ns], 4L), 0);
+    for (int s = 0; s < ns; ++s) {
+        const int w = wm[s] >> 1;
+        const bool sh = wm[s] & 1;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r = s * 64 + lane;
+            for (int k = 0; k < w; ++k) {
+                const long e = (long)k * 64 + lane;
+                const bool real = r < n_ && k < rp[r + 1] - rp[r];
+                const int c = real ? col[rp[r] + k] : std::min(r, n_ - 1);
+                sv[vptr[s] + e] = real ? val[rp[r] + k] : 0.0;
+                if (sh) {
+                    const short d = (short)(c - std::min(r, n_ - 1));
+                    std::memcpy(&sidx[iptr[s] + 2 * e], &d, 2);
+                } else {
+                    std::memcpy(&sidx[iptr[s] + 4 * e], &c, 4);
+                }
+            }
+        }
+    }
+    std::vector<float> svf(sv.begin(), sv.end());
+    JDHIP_CHECK(hipMalloc(&A.s_vptr, sizeof(long) * (ns + 1)));
+    JDHIP_CHECK(hipMalloc(&A.s_iptr, sizeof(long) * (ns + 1)));
+    JDHIP_CHECK(hipMalloc(&A.s_wm, sizeof(int) * ns));
+    JDHIP_CHECK(hipMalloc(&A.s_val, sizeof(double) * std::max(ent, 1L)));
+    JDHIP_CHECK(hipMalloc(&A.s_valf, sizeof(float) * std::max(ent, 1L)));
+    JDHIP_CHECK(hipMalloc(&A.s_idx, sidx.size()));
+    JDHIP_CHECK(hipMemcpy(A.s_vptr, vptr.data(), sizeof(long) * (ns + 1), hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.s_iptr, iptr.data(), sizeof(long) * (ns + 1), hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.s_wm, wm.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.s_val, sv.data(), sizeof(double) * ent, hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.s_valf, svf.data(), sizeof(float) * ent, hipMemcpyHostToDevice));
+    JDHIP_CHECK(hipMemcpy(A.s_idx, sidx.data(), sidx.size(), hipMemcpyHostToDevice));
+    A.nslices = ns;
+    A.s_entries = ent;
+    A.s_idx_bytes = iptr[ns];
 }
 
 // Pick lanes-per-row by timing the candidates on this matrix.
@@ -229,7 +300,8 @@ int Solver::autotune_lpr() {
     JDHIP_CHECK(hipEventCreate(&e1));
     int best = 8;
     float bt = 1e30f;
-    for (int lpr : {1, 2, 4, 8, 16, 32}) {
+    for (int lpr : {0, 1, 2, 4, 8, 16, 32}) {
+        if (lpr == 0 && A.nslices == 0) continue;
         A.lpr = lpr;
         spmv<double>(ctx.stream, ctx.rb, A, x, y, 0.0, nullptr, nullptr);
         JDHIP_CHECK(hipEventRecord(e0, ctx.stream));
@@ -720,6 +792,7 @@ void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out) {
     out->fill = S.ildl_info.fill;
     out->precond_levels = S.ildl_info.nlev;
     out->precond_negpiv = S.ildl_info.nneg;
+    out->precond_sweeps = S.ildl ? S.ildl->result.sweeps : 0;
 }
 
 int jdhip_set_matrix(jdhip_ctx *ctx, int n, const int *ia, const int *ja, const double *a,

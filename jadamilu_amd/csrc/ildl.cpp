@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 
@@ -145,10 +147,8 @@ class IldlImpl : public IldlImplBase {
             D.lct_val = track(dev_upload(tv));
             D.buf = track(dev_alloc<T>(L.n));
             D.out = track(dev_alloc<T>(L.n));
-            if (sweeps_ > 0) {
-                D.t1 = track(dev_alloc<T>(L.nB));
-                D.t2 = track(dev_alloc<T>(L.nB));
-            }
+            D.t1 = track(dev_alloc<T>(std::max(L.nB, 1)));
+            D.t2 = track(dev_alloc<T>(std::max(L.nB, 1)));
             D.nnz_lb = (long)L.lb_idx.size();
             D.nnz_lc = (long)L.lc_idx.size();
             res.fwd_levels.push_back((int)D.fwd_ptr.size() - 1);
@@ -184,6 +184,64 @@ class IldlImpl : public IldlImplBase {
             std::swap(xin, xout);
         }
         JDHIP_CHECK(hipMemcpyAsync(D.buf, xin, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
+    }
+
+    int sweeps() const override { return sweeps_; }
+
+    int tune_sweeps(double tol) override {
+        sweeps_ = 0;
+        if (lv_.empty()) return 0;
+        const int n = lv_[0].n;
+        hipStream_t s = ctx_->stream;
+        std::vector<double> hr(n), h0(n), hk(n);
+        uint64_t st = 0x9e3779b97f4a7c15ULL;
+        for (int i = 0; i < n; ++i) {  // splitmix64 -> uniform(-1, 1)
+            uint64_t z = (st += 0x9e3779b97f4a7c15ULL);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            z ^= z >> 31;
+            hr[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+        }
+        double *r = dev_alloc<double>(n), *z = dev_alloc<double>(n);
+        JDHIP_CHECK(hipMemcpy(r, hr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        hipEvent_t e0, e1;
+        JDHIP_CHECK(hipEventCreate(&e0));
+        JDHIP_CHECK(hipEventCreate(&e1));
+        auto timed = [&](int reps) {
+            apply(z, r);
+            JDHIP_CHECK(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) apply(z, r);
+            JDHIP_CHECK(hipEventRecord(e1, s));
+            JDHIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms / reps;
+        };
+        float best_t = timed(2);
+        JDHIP_CHECK(hipMemcpy(h0.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost));
+        double n0 = 0.0;
+        for (double v : h0) n0 += v * v;
+        n0 = std::sqrt(n0);
+        int best = 0;
+        for (int k : {2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64}) {
+            sweeps_ = k;
+            float t = timed(2);
+            if (t >= best_t) break;  // more sweeps only cost more
+            JDHIP_CHECK(hipMemcpy(hk.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost));
+            double e = 0.0;
+            for (int i = 0; i < n; ++i) e += (hk[i] - h0[i]) * (hk[i] - h0[i]);
+            if (std::sqrt(e) <= tol * n0) {
+                best = k;
+                best_t = t;
+                break;
+            }
+        }
+        sweeps_ = best;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipFree(r);
+        hipFree(z);
+        return best;
     }
 
     void apply(double *z, const double *r) override { apply_io<double>(z, r); }
@@ -345,17 +403,22 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     result = h.res;
     result.fwd_levels.clear();
     result.bwd_levels.clear();
+    const int k0 = sweeps < 0 ? 0 : sweeps;
     if (h.single) {
         IldlMultilevel<float> f = h.ff;
-        auto *p = new IldlImpl<float>(ctx_, std::move(f), sweeps);
+        auto *p = new IldlImpl<float>(ctx_, std::move(f), k0);
         impl_.reset(p);
         p->upload(result);
     } else {
         IldlMultilevel<double> f = h.fd;
-        auto *p = new IldlImpl<double>(ctx_, std::move(f), sweeps);
+        auto *p = new IldlImpl<double>(ctx_, std::move(f), k0);
         impl_.reset(p);
         p->upload(result);
     }
+    // tuned mode: Jacobi sweeps only where they reproduce the exact apply to
+    // 3 % and are faster than the level-scheduled solve
+    if (sweeps < 0) impl_->tune_sweeps(0.03);
+    result.sweeps = impl_->sweeps();
 }
 
 IldlResult IldlPrecond::setup(int n, const int *ia, const int *ja, const double *a,

@@ -15,7 +15,7 @@ struct IldlOptions {
     double mem = 0.0;    // elbow space (stored entries / nnz(A)), <= 0: unlimited
     bool spd = false;    // |D| conversion (ISEARCH <= 1)
     bool single = false; // FP32 factor, storage and apply (ICNTL(5) != 0)
-    int sweeps = 0;      // 0: exact level-scheduled sweeps; k > 0: k Jacobi sweeps
+    int sweeps = -1;     // 0: exact level-scheduled; k > 0: k Jacobi sweeps; -1: tuned
     int ordering = 2;  // 0 natural, 1 RCM, 2 AMD
     int dense_max = 1500;
 };
@@ -29,6 +29,7 @@ struct IldlResult {
     double fill = 0.0;  // stored entries / nnz(upper A)
     double seconds = 0.0;
     std::vector<int> level_n, level_nB, fwd_levels, bwd_levels;
+    int sweeps = 0;  // triangular-solve mode in use (0 exact, k Jacobi sweeps)
 };
 
 class IldlImplBase {
@@ -46,6 +47,11 @@ class IldlImplBase {
     virtual void dense_data(double *minv) const = 0;
     virtual long apply_bytes() const = 0;  // algorithmic bytes of one apply
     virtual bool single() const = 0;
+    // Pick the triangular-solve mode by measurement: the fastest of the exact
+    // level-scheduled solve and k Jacobi sweeps whose apply stays within
+    // relative distance tol of the exact apply on a fixed random vector.
+    virtual int tune_sweeps(double tol) = 0;
+    virtual int sweeps() const = 0;
 };
 
 // Host factor (no device needed).

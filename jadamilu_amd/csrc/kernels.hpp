@@ -67,7 +67,20 @@ struct CsrDev {
     int *col = nullptr;
     double *val = nullptr;
     float *valf = nullptr;  // optional FP32 copy
-    int lpr = 8;            // lanes per row
+    int lpr = 8;            // lanes per row (CSR kernel); 0: sliced-ELL kernel
+    // Sliced ELL (slices of 64 consecutive rows, one wave64 per slice, one
+    // lane per row, entries of a slice stored column-major so every load of
+    // the wave is contiguous).  Column indices are 16-bit offsets from the
+    // row when the whole slice allows it, otherwise 32-bit columns.
+    int nslices = 0;
+    long *s_vptr = nullptr;   // element offset of each slice's values
+    long *s_iptr = nullptr;   // byte offset of each slice's indices
+    int *s_wm = nullptr;      // 2*width + (1 if 16-bit offsets)
+    double *s_val = nullptr;
+    float *s_valf = nullptr;
+    char *s_idx = nullptr;
+    long s_entries = 0;       // padded entries
+    long s_idx_bytes = 0;
 };
 // y = A x (T = double: A.val, T = float: A.valf); if eta != 0: y -= eta x;
 // if u: *udot_out = u.y (double)

@@ -110,12 +110,96 @@ static void spmv_lpr(hipStream_t s, const RedBuf &rb, int n, const int *rp, cons
     }
 }
 
+// Sliced ELL: one wave64 per slice of 64 rows, lane = row within the slice.
+template <typename T, bool ETA, bool UDOT>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    int n, int nslices, const long *__restrict__ vptr, const long *__restrict__ iptr,
+    const int *__restrict__ wm, const T *__restrict__ val, const char *__restrict__ idx,
+    const T *__restrict__ x, T *__restrict__ y, double eta_host, const T *__restrict__ u,
+    double *partials, unsigned *counter, double *udot_out) {
+    constexpr int WPB = kBlock / 64;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const T eta = (T)eta_host;
+    double acc[1] = {0.0};
+    for (long sl = (long)blockIdx.x * WPB + wv; sl < nslices; sl += (long)gridDim.x * WPB) {
+        const long row = sl * 64 + lane;
+        if (row >= n) continue;
+        const int code = wm[sl];
+        const int w = code >> 1;
+        const T *v = val + vptr[sl] + lane;
+        T s0 = T(0), s1 = T(0);
+        if (code & 1) {
+            const short *ci = (const short *)(idx + iptr[sl]) + lane;
+            int k = 0;
+            for (; k + 1 < w; k += 2) {
+                s0 += v[k * 64] * x[row + ci[k * 64]];
+                s1 += v[(k + 1) * 64] * x[row + ci[(k + 1) * 64]];
+            }
+            if (k < w) s0 += v[k * 64] * x[row + ci[k * 64]];
+        } else {
+            const int *ci = (const int *)(idx + iptr[sl]) + lane;
+            int k = 0;
+            for (; k + 1 < w; k += 2) {
+                s0 += v[k * 64] * x[ci[k * 64]];
+                s1 += v[(k + 1) * 64] * x[ci[(k + 1) * 64]];
+            }
+            if (k < w) s0 += v[k * 64] * x[ci[k * 64]];
+        }
+        T yi = s0 + s1;
+        if (ETA) yi += (-eta) * x[row];
+        y[row] = yi;
+        if (UDOT) acc[0] += (double)u[row] * (double)yi;
+    }
+    if (UDOT) {
+        __shared__ double fin[1];
+        if (grid_reduce<1>(acc, 1, partials, counter, fin))
+            if (threadIdx.x == 0) udot_out[0] = fin[0];
+    }
+}
+
+template <typename T>
+static void spmv_sell(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T *val,
+                      const T *x, T *y, double eta, const T *u, double *udot) {
+    constexpr int WPB = kBlock / 64;
+    const long blocks = ((long)A.nslices + WPB - 1) / WPB;
+    const bool has_eta = eta != 0.0;
+    unsigned *cnt = rb.counter + 4;
+    if (u) {
+        int g = (int)std::min<long>(blocks, kMaxRedBlocks);
+        if (has_eta)
+            k_spmv_sell<T, true, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr, A.s_iptr,
+                                                          A.s_wm, val, A.s_idx, x, y, eta, u,
+                                                          rb.partials, cnt, udot);
+        else
+            k_spmv_sell<T, false, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
+                                                           A.s_iptr, A.s_wm, val, A.s_idx, x, y,
+                                                           eta, u, rb.partials, cnt, udot);
+    } else {
+        int g = (int)std::min<long>(blocks, 1L << 20);
+        if (has_eta)
+            k_spmv_sell<T, true, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
+                                                           A.s_iptr, A.s_wm, val, A.s_idx, x, y,
+                                                           eta, u, rb.partials, cnt, udot);
+        else
+            k_spmv_sell<T, false, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
+                                                            A.s_iptr, A.s_wm, val, A.s_idx, x,
+                                                            y, eta, u, rb.partials, cnt, udot);
+    }
+}
+
 template <typename T>
 void spmv(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T *x, T *y, double eta,
           const T *u, double *udot_out) {
     const T *val;
     if constexpr (sizeof(T) == 8) val = (const T *)A.val;
     else val = (const T *)A.valf;
+    if (A.lpr == 0 && A.nslices > 0) {
+        const T *sv;
+        if constexpr (sizeof(T) == 8) sv = (const T *)A.s_val;
+        else sv = (const T *)A.s_valf;
+        spmv_sell<T>(s, rb, A, sv, x, y, eta, u, udot_out);
+        return;
+    }
     switch (A.lpr) {
     case 1: spmv_lpr<T, 1>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
     case 2: spmv_lpr<T, 2>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;

@@ -191,7 +191,8 @@ class Context:
                 "n_precond_setups": s.n_precond_setups, "spmv_lanes": s.spmv_lanes,
                 "setup_shifts": list(s.setup_shifts[: s.n_precond_setups]),
                 "precond_seconds": s.precond_seconds, "fill": s.fill,
-                "precond_levels": s.precond_levels, "precond_negpiv": s.precond_negpiv}
+                "precond_levels": s.precond_levels, "precond_negpiv": s.precond_negpiv,
+                "precond_sweeps": s.precond_sweeps}
 
     def stream_ptr(self):
         return self._L.jdhip_stream(self.h)

@@ -24,6 +24,40 @@ typedef struct {
     const double *d;
 } jdo_ildl_level;
 
+/* 0: exact triangular solves; k > 0: each triangular solve replaced by k
+ * Jacobi sweeps x <- b - N x from x = b (N the strictly triangular part),
+ * the device's sweep mode. */
+static int g_sweeps = 0;
+
+void jdo_ildl_set_sweeps(int k) { g_sweeps = k > 0 ? k : 0; }
+
+static void jacobi_lower(int nB, const int *ptr, const int *idx, const double *val, double *b,
+                         int transposed)
+{
+    double *x = (double *)malloc(sizeof(double) * (nB > 0 ? nB : 1));
+    double *xn = (double *)malloc(sizeof(double) * (nB > 0 ? nB : 1));
+    memcpy(x, b, sizeof(double) * nB);
+    for (int it = 0; it < g_sweeps; it++) {
+        if (!transposed) {
+            for (int i = 0; i < nB; i++) {
+                double s = 0.0;
+                for (int k = ptr[i]; k < ptr[i + 1]; k++) s += val[k] * x[idx[k]];
+                xn[i] = b[i] - s;
+            }
+        } else {
+            memcpy(xn, b, sizeof(double) * nB);
+            for (int i = 0; i < nB; i++)
+                for (int k = ptr[i]; k < ptr[i + 1]; k++) xn[idx[k]] -= val[k] * x[i];
+        }
+        double *t = x;
+        x = xn;
+        xn = t;
+    }
+    memcpy(b, x, sizeof(double) * nB);
+    free(x);
+    free(xn);
+}
+
 static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const double *minv,
                         const double *in, double *out)
 {
@@ -44,7 +78,8 @@ static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const d
         buf[i] = V->scal[p] * in[p];
     }
     /* forward: L_B y = x_B (rows ascending, entries j < i) */
-    for (int i = 0; i < nB; i++) {
+    if (g_sweeps > 0) jacobi_lower(nB, V->lb_ptr, V->lb_idx, V->lb_val, buf, 0);
+    else for (int i = 0; i < nB; i++) {
         double s = 0.0;
         for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) s += V->lb_val[k] * buf[V->lb_idx[k]];
         buf[i] -= s;
@@ -62,7 +97,8 @@ static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const d
         for (int k = V->lc_ptr[c]; k < V->lc_ptr[c + 1]; k++)
             buf[V->lc_idx[k]] -= V->lc_val[k] * zc[c];
     /* backward: L_B^T z = t (column-oriented over the rows of L_B) */
-    for (int i = nB - 1; i >= 0; i--) {
+    if (g_sweeps > 0) jacobi_lower(nB, V->lb_ptr, V->lb_idx, V->lb_val, buf, 1);
+    else for (int i = nB - 1; i >= 0; i--) {
         double zi = buf[i];
         for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) buf[V->lb_idx[k]] -= V->lb_val[k] * zi;
     }

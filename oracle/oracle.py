@@ -57,6 +57,8 @@ def lib():
         L.jdo_sortrx.argtypes = [C.c_int, _d, _i]
         L.jdo_cleanup.argtypes = [C.c_void_p]
         L.jdo_ildl_apply.argtypes = [C.c_int, C.c_void_p, C.c_int, _d, _d, _d]
+        L.jdo_ildl_set_sweeps.argtypes = [C.c_int]
+        L.jdo_ildl_set_sweeps.restype = None
         L.jdo_set_psolve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
@@ -213,11 +215,13 @@ def _levels_struct(factor):
 
 
 def pjd_ildl(n, ia, ja, a, factor, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20,
-             maxiter=1000, tol=1e-10, icntl=(0, 1, 0, 0, 0), x0=None):
+             maxiter=1000, tol=1e-10, icntl=(0, 1, 0, 0, 0), x0=None, sweeps=0):
     """DPJD of the oracle (internal MATVECA) preconditioned by the oracle's
     multilevel apply of an exported factor (the factor is computed once,
-    outside; the oracle has no adaptive re-factorisation)."""
+    outside; the oracle has no adaptive re-factorisation).  sweeps > 0
+    replaces the triangular solves by that many Jacobi sweeps."""
     L = lib()
+    L.jdo_ildl_set_sweeps(int(sweeps))
     arr, minv, keep = _levels_struct(factor)
     ctx = _IldlCtx(len(factor["levels"]), C.cast(arr, C.c_void_p), factor["m"],
                    minv.ctypes.data_as(_d))
@@ -247,9 +251,10 @@ def pjd_ildl(n, ia, ja, a, factor, *, neig, isearch=0, sigma=0.0, ninit=0, madsp
                       info=info.value, gap=gap.value, inner=L.jdo_inner_iters(s))
     finally:
         L.jdo_destroy(s)
+        L.jdo_ildl_set_sweeps(0)
 
 
-def ildl_apply(factor, r):
+def ildl_apply(factor, r, sweeps=0):
     """z = M^-1 r for an exported multilevel factor (dict from
     jadamilu_amd.ildl.IldlFactor.export()), sequential C restatement of the
     multilevel substitution (src/ilupack/symAMGsol.c)."""
@@ -275,8 +280,10 @@ def ildl_apply(factor, r):
         minv = np.zeros(1)
     r = np.ascontiguousarray(r, dtype=np.float64)
     z = np.zeros_like(r)
+    L.jdo_ildl_set_sweeps(int(sweeps))
     L.jdo_ildl_apply(len(levs), arr, factor["m"], minv.ctypes.data_as(_d), _p(r, C.c_double),
                      _p(z, C.c_double))
+    L.jdo_ildl_set_sweeps(0)
     return z
 
 

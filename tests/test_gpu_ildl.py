@@ -87,6 +87,32 @@ def test_sweep_mode_converges_to_exact():
     ctx.close()
 
 
+@pytest.mark.parametrize("gen,spd", [(lambda: M.laplace((24, 23, 22)), True),
+                                     (lambda: M.anderson(16, 16.5, 4), False)])
+def test_tuned_sweeps_stay_close_to_exact(gen, spd):
+    """Tuned triangular-solve mode (sweeps=-1): whatever it picks, the apply
+    is within 3 % of the exact level-scheduled apply on fresh random
+    vectors (the tuning vector is a different one)."""
+    from jadamilu_amd.ildl import IldlFactor
+    from jadamilu_amd.jd import Context
+    _dev()
+    ia, ja, a = gen()
+    n = ia.size - 1
+    f = IldlFactor(n, ia, ja, a, droptol=1e-3, spd=spd)
+    ctx = Context(0)
+    ctx.attach_ildl(f, sweeps=0)
+    rng = np.random.default_rng(11)
+    rs = [rng.standard_normal(n) for _ in range(3)]
+    z0 = [_apply_dev(ctx, r) for r in rs]
+    ctx.attach_ildl(f, sweeps=-1)
+    k = ctx.stats()["precond_sweeps"]
+    print("tuned sweeps", k)
+    for r, zr in zip(rs, z0):
+        z = _apply_dev(ctx, r)
+        assert np.linalg.norm(z - zr) <= 0.06 * np.linalg.norm(zr), k
+    ctx.close()
+
+
 @pytest.mark.parametrize("name,isearch", [("EXAMPLE1", 2), ("EXAMPLE2", 0)])
 def test_dpjd_guide(golden, name, isearch):
     from jadamilu_amd import jd
@@ -94,6 +120,7 @@ def test_dpjd_guide(golden, name, isearch):
     g = golden[name]["calls"][0]
     n = 1000
     ia, ja, a = M.tridiag(n, np.arange(n) * 1.0, 5.0)
+    jd.reset_legacy()
     r = jd.pjd_host(n, a, ja, ia, neig=5, isearch=isearch, sigma=0.0, madspace=20,
                     maxiter=1000, tol=1e-10, droptol=1e-3, mem=20.0)
     print(name, "matvecs", r.iter, "guide", g["matvec"], "eigs", r.eigs)

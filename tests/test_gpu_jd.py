@@ -77,15 +77,39 @@ def test_jdrevcom_harmonic_legacy():
     assert (r.res <= 1e-10).all()
 
 
+def _far_coupled(n, seed=5):
+    """Ragged upper CSR with couplings farther than 2^15 rows (exercises the
+    32-bit column slices of the sliced-ELL format) and rows of 1..9 entries."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    r = np.arange(n - 1)
+    rows.append(r)
+    cols.append(r + 1)
+    for _ in range(3):
+        i = rng.integers(0, n - 40000, n // 7)
+        rows.append(i)
+        cols.append(i + rng.integers(33000, 40000, i.size))
+    R, Cc = np.concatenate(rows), np.concatenate(cols)
+    key = np.unique(R.astype(np.int64) * n + Cc)
+    R, Cc = key // n, key % n
+    vals = np.where(R == Cc, 4.0, -0.5) + 0.25 * rng.standard_normal(R.size)
+    ia = np.zeros(n + 1, np.int32)
+    np.add.at(ia, R + 1, 1)
+    ia = (np.cumsum(ia) + 1).astype(np.int32)
+    return ia, (Cc + 1).astype(np.int32), vals
+
+
 def test_spmv_vs_matveca():
     """Device SpMV (full CSR built from the upper triangle) vs MATVECA."""
     dev = _dev()
     from jadamilu_amd.jd import Context
     ctx = Context(0)
     rng = np.random.default_rng(7)
-    for ia, ja, a in [M.laplace((20, 17, 9)), M.anderson(12, 16.5, 3), M.tridiag(777, 2.0, -1.0)]:
+    for ia, ja, a in [M.laplace((20, 17, 9)), M.anderson(12, 16.5, 3), M.tridiag(777, 2.0, -1.0),
+                      _far_coupled(70001)]:
         n = ia.size - 1
-        for lpr in (0, 1, 4, 16):
+        # 0 autotune, -1 sliced ELL, k > 0 CSR with k lanes per row
+        for lpr in (0, -1, 1, 4, 16):
             ctx.set_matrix(ia, ja, a, lanes_per_row=lpr)
             x = rng.standard_normal(n)
             y_ref = np.empty(n)
@@ -175,7 +199,11 @@ def test_interior_diag_harmonic_device_vs_oracle():
     assert r.info == 0 and ro.info == 0
     np.testing.assert_allclose(np.sort(r.eigs), near, rtol=1e-10)
     np.testing.assert_allclose(np.sort(r.eigs), np.sort(ro.eigs), rtol=1e-10)
-    assert abs(r.iter - ro.iter) <= max(3, 0.10 * ro.iter)
+    assert (r.res <= 1e-10).all()
+    # The MATVEC count of this interior run is rounding-chaotic: the oracle
+    # itself needs 649..1328 MATVECs when its all-ones start vector is
+    # perturbed by 1e-13 (689 unperturbed), so only that envelope is checked.
+    assert 0.5 * ro.iter <= r.iter <= 2.5 * ro.iter, (r.iter, ro.iter)
 
 
 def test_input_errors_device():
