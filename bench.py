@@ -60,10 +60,21 @@ def spmv_bytes(n, nnz_u, s=8):
     return (s + 4) * nnz_u + 4 * (n + 1) + 2 * s * n
 
 
-def spmv_probe(ctx, n, nnz_u, reps, dev, torch):
-    """reps SpMVs on the library stream, then reps copies of a 256 MiB
-    buffer (torch copy kernel, 268,435,456 bytes read and written): the
-    copy's FETCH_SIZE/WRITE_SIZE per dispatch calibrate the counters."""
+def kernel_probe(ctx, n, nnz_u, reps, dev, torch, sweep=None):
+    """reps SpMVs on the library stream, reps launches of the first-level
+    Jacobi sweep when sweep = (ia, ja, a, spd, single, k) is given, then reps
+    copies of a 256 MiB buffer (torch copy kernel, 268,435,456 bytes read and
+    written): the copy's FETCH_SIZE/WRITE_SIZE per dispatch calibrate the
+    counters."""
+    out = {}
+    if sweep is not None:
+        from jadamilu_amd.ildl import IldlFactor
+        ia, ja, a, spd, single, k = sweep
+        f = IldlFactor(n, ia, ja, a, shift=0.0, droptol=1e-3, spd=spd, single=single)
+        ctx.attach_ildl(f, sweeps=k)
+        ms, nb = ctx.ildl_time_sweep(reps)
+        out.update({"sweep_avg_ms": ms, "sweep_alg_bytes": nb,
+                    "sweeps_chosen": ctx.stats()["precond_sweeps"]})
     xs = torch.ones(n, dtype=torch.float64, device=dev)
     ys = torch.empty(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
@@ -87,9 +98,10 @@ def spmv_probe(ctx, n, nnz_u, reps, dev, torch):
     c1.record()
     c1.synchronize()
     copy_ms = c0.elapsed_time(c1) / reps
-    print(json.dumps({"spmv_avg_ms": spmv_ms, "spmv_alg_bytes": spmv_bytes(n, nnz_u),
-                      "copy_avg_ms": copy_ms, "copy_bytes_each_way": 1 << 28,
-                      "copy_GBps_read_plus_write": 2 * (1 << 28) / (copy_ms * 1e-3) / 1e9}))
+    out.update({"spmv_avg_ms": spmv_ms, "spmv_alg_bytes": spmv_bytes(n, nnz_u),
+                "copy_avg_ms": copy_ms, "copy_bytes_each_way": 1 << 28,
+                "copy_GBps_read_plus_write": 2 * (1 << 28) / (copy_ms * 1e-3) / 1e9})
+    print(json.dumps(out))
 
 
 def main():
@@ -111,9 +123,10 @@ def main():
                     help="ILDL triangular solves: 0 exact level-scheduled, k Jacobi sweeps, "
                          "-1 tuned at set-up")
     ap.add_argument("--single", action="store_true", help="FP32 ILDL factor (ICNTL(5)=1)")
-    ap.add_argument("--spmv-only", action="store_true",
-                    help="only the SpMV of the workload (and a plain device copy of known "
-                         "size for calibrating HBM counters), for counter collection")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the SpMV and the first-level ILDL sweep of the workload (and a "
+                         "plain device copy of known size for calibrating HBM counters), "
+                         "for kernel-level profiles")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -142,8 +155,11 @@ def main():
     X = torch.zeros(lx, dtype=torch.float64, device=dev)
     x0_dev = torch.tensor(x0, device=dev)
     torch.cuda.synchronize()
-    if args.spmv_only:
-        spmv_probe(ctx, n, nnz_u, args.spmv_reps, dev, torch)
+    if args.kernels_only:
+        sw = None
+        if args.precond == "ildl":
+            sw = (ia, ja, a, isearch <= 1, args.single, args.sweeps)
+        kernel_probe(ctx, n, nnz_u, args.spmv_reps, dev, torch, sweep=sw)
         ctx.close()
         return
 
@@ -216,13 +232,29 @@ def main():
     e1.record(stream)
     e1.synchronize()
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
-    alg = spmv_bytes(n, nnz_u)
+    alg = spmv_bytes(n, nnz_u, 8)
     achieved = alg / (spmv_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"spmv_traffic_{args.workload}.json")
+    traffic = {}
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f)
+    roof_spmv = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get("spmv"),
+                 "kernel": "k_spmv_sell (symmetric SpMV, sliced ELL)" if r.stats.get(
+                     "spmv_lanes") == 0 else "k_spmv (symmetric SpMV, CSR)",
+                 "alg_bytes": alg, "avg_ms": spmv_ms}
+    roofline = roof_spmv
+    if args.precond == "ildl" and k_sw > 0:
+        # the dominant kernel of the solve: one Jacobi sweep of the
+        # preconditioner's first level (timed with HIP events on the library
+        # stream, same launches as inside the solve)
+        sw_ms, sw_bytes = ctx.ildl_time_sweep(args.spmv_reps)
+        sw_ach = sw_bytes / (sw_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": sw_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": sw_ach / HBM_PEAK_GBS, "traffic": traffic.get("sweep"),
+                    "kernel": "k_jacobi_sell (ILDL level-1 Jacobi sweep, SELL-C-sigma)",
+                    "alg_bytes": sw_bytes, "avg_ms": sw_ms}
 
     # ---- CPU baseline: oracle on a bounded sample of the same solve ----
     cpu = None
@@ -288,10 +320,8 @@ def main():
                                f"triangular solves: {tri_mode})"),
                    "icntl": list(icntl),
                    "start": "NINIT=1, numpy PCG64 seed 0 uniform[0,1)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_spmv (symmetric SpMV, full CSR)", "alg_bytes": alg,
-                     "avg_ms": spmv_ms},
+        "roofline": roofline,
+        "roofline_spmv": roof_spmv,
         "cpu_baseline": cpu,
         "result": {"neig": r.neig, "info": r.info, "matvecs": r.iter,
                    "inner_iters": r.stats.get("inner_iters"),

@@ -166,6 +166,11 @@ int jdhip_ildl_stats(jdhip_ildl *f, double *fill, int *nneg, double *condest, do
 int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps);
 /* dependency levels of the uploaded factor and algorithmic bytes per apply */
 int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels, long *apply_bytes);
+/* Diagnostic: time reps launches of one Jacobi sweep over the first level's
+ * L_B (the largest single kernel of the preconditioner apply) on ctx's stream
+ * with HIP events; *alg_bytes = its algorithmic bytes per launch.  No
+ * reference counterpart.  0 on success, -1 without an ILDL preconditioner. */
+int jdhip_ildl_time_sweep(jdhip_ctx *ctx, int reps, double *avg_ms, long *alg_bytes);
 
 /* ---- eigenvalue adjoints (no reference counterpart) ---------------------- */
 /* Gradient of L = sum_k w[k] lambda_k (simple, gap-separated eigenvalues,

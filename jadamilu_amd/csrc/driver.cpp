@@ -963,6 +963,17 @@ int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels,
     return 0;
 }
 
+int jdhip_ildl_time_sweep(jdhip_ctx *ctx, int reps, double *avg_ms, long *alg_bytes) {
+    try {
+        Solver &S = *ctx->S;
+        if (!S.ildl || !S.ildl->ready()) return -1;
+        return S.ildl->impl()->time_sweep(reps, avg_ms, alg_bytes);
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -2;
+    }
+}
+
 }  // extern "C"
 
 struct jdhip_ildl {

@@ -84,6 +84,61 @@ void schedule(int n, const std::vector<int> &ptr, const std::vector<int> &idx, b
     for (int i = 0; i < n; ++i) rows[f[lvl[i]]++] = i;
 }
 
+// SELL-C-sigma copy of a CSR (C = 64 rows per slice, rows sorted by length
+// inside windows of kSigma rows) for the Jacobi sweeps.
+constexpr int kSigma = 1024;
+
+template <typename T>
+struct SellHost {
+    std::vector<long> vptr;
+    std::vector<int> wid, rowid, col;
+    std::vector<T> val;
+};
+
+template <typename T>
+SellHost<T> build_sell(int nrows, const std::vector<int> &ptr, const std::vector<int> &idx,
+                       const std::vector<T> &val) {
+    SellHost<T> h;
+    std::vector<int> order(nrows);
+    for (int w0 = 0; w0 < nrows; w0 += kSigma) {
+        const int w1 = std::min(nrows, w0 + kSigma);
+        for (int r = w0; r < w1; ++r) order[r] = r;
+        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+            return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b];
+        });
+        for (int s0 = w0; s0 < w1; s0 += 64) {
+            int w = 0;
+            for (int r = s0; r < std::min(w1, s0 + 64); ++r)
+                w = std::max(w, ptr[order[r] + 1] - ptr[order[r]]);
+            const long base = h.val.size();
+            h.vptr.push_back(base);
+            h.wid.push_back(w);
+            h.val.resize(base + 64L * w, T(0));
+            h.col.resize(base + 64L * w, 0);
+            for (int lane = 0; lane < 64; ++lane) {
+                const int t = s0 + lane;
+                const int row = (t < w1) ? order[t] : -1;
+                h.rowid.push_back(row);
+                if (row < 0) continue;
+                for (int k = 0; k < ptr[row + 1] - ptr[row]; ++k) {
+                    h.val[base + (long)k * 64 + lane] = val[ptr[row] + k];
+                    h.col[base + (long)k * 64 + lane] = idx[ptr[row] + k];
+                }
+            }
+        }
+    }
+    h.vptr.push_back((long)h.val.size());
+    return h;
+}
+
+template <typename T>
+struct SellDev {
+    int nslices = 0;
+    long *vptr = nullptr;
+    int *wid = nullptr, *rowid = nullptr, *col = nullptr;
+    T *val = nullptr;
+};
+
 template <typename T>
 struct DevLevel {
     int n = 0, nB = 0, nC = 0;
@@ -101,6 +156,8 @@ struct DevLevel {
     int *fwd_rows = nullptr, *bwd_rows = nullptr;
     T *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;
     long nnz_lb = 0, nnz_lc = 0;
+    SellDev<T> sf, sb;  // sweep copies of L_B (forward) and L_B^T (backward)
+    SellDev<T> sc, sct; // L_CB and L_CB^T
 };
 
 template <typename T>
@@ -113,53 +170,127 @@ class IldlImpl : public IldlImplBase {
         for (void *p : allocs_) hipFree(p);
     }
 
+    // Upload with a storage renumbering per level: the B and the C positions
+    // are each stored in the order of the original matrix index they stand
+    // for, so gathers, scatters and the sweeps' x reads follow the locality
+    // of the input (the factor's own order is the fill-reducing one, which
+    // scatters neighbours).  Dependency levels are computed in the factor's
+    // triangular order and relabelled; the host copy f_ keeps the factor's
+    // numbering for export.
     void upload(IldlResult &res) {
-        for (auto &L : f_.levels) {
+        const int nlev = (int)f_.levels.size();
+        std::vector<long> orig;  // original index of each (relabelled) input index
+        std::vector<int> cnew;   // previous level: old C index -> new C index
+        std::vector<int> qC_last;
+        for (int l = 0; l < nlev; ++l) {
+            const IldlLevel<T> &L = f_.levels[l];
+            const int n = L.n, nB = L.nB, nC = n - nB;
+            if (l == 0) {
+                orig.resize(n);
+                for (int i = 0; i < n; ++i) orig[i] = i;
+            }
+            std::vector<int> perm(n);
+            std::vector<T> scal(n);
+            for (int i = 0; i < n; ++i) perm[i] = cnew.empty() ? L.perm[i] : cnew[L.perm[i]];
+            for (int j = 0; j < n; ++j) scal[cnew.empty() ? j : cnew[j]] = L.scal[j];
+            std::vector<int> q(n), qinv(n);
+            for (int i = 0; i < n; ++i) q[i] = i;
+            auto by_orig = [&](int a, int b) { return orig[perm[a]] < orig[perm[b]]; };
+            std::stable_sort(q.begin(), q.begin() + nB, by_orig);
+            std::stable_sort(q.begin() + nB, q.end(), by_orig);
+            for (int t = 0; t < n; ++t) qinv[q[t]] = t;
+
             DevLevel<T> D;
-            D.n = L.n;
-            D.nB = L.nB;
-            D.nC = L.n - L.nB;
-            D.perm = track(dev_upload(L.perm));
-            D.scal = track(dev_upload(L.scal));
-            std::vector<T> dinv(L.nB);
-            for (int i = 0; i < L.nB; ++i) dinv[i] = T(1) / L.d[i];
+            D.n = n;
+            D.nB = nB;
+            D.nC = nC;
+            std::vector<int> permN(n);
+            for (int t = 0; t < n; ++t) permN[t] = perm[q[t]];
+            D.perm = track(dev_upload(permN));
+            D.scal = track(dev_upload(scal));
+            std::vector<T> dinv(nB);
+            for (int t = 0; t < nB; ++t) dinv[t] = T(1) / L.d[q[t]];
             D.dinv = track(dev_upload(dinv));
-            D.lb_ptr = track(dev_upload(L.lb_ptr));
-            D.lb_idx = track(dev_upload(L.lb_idx));
-            D.lb_val = track(dev_upload(L.lb_val));
+            // L_B and L_CB in the storage numbering
+            std::vector<int> bp(nB + 1, 0), bi, cp(nC + 1, 0), ci;
+            std::vector<T> bv, cv;
+            bi.reserve(L.lb_idx.size());
+            bv.reserve(L.lb_idx.size());
+            for (int t = 0; t < nB; ++t) {
+                const int i = q[t];
+                for (int k = L.lb_ptr[i]; k < L.lb_ptr[i + 1]; ++k) {
+                    bi.push_back(qinv[L.lb_idx[k]]);
+                    bv.push_back(L.lb_val[k]);
+                }
+                bp[t + 1] = (int)bi.size();
+            }
+            ci.reserve(L.lc_idx.size());
+            cv.reserve(L.lc_idx.size());
+            for (int c = 0; c < nC; ++c) {
+                const int oc = q[nB + c] - nB;
+                for (int k = L.lc_ptr[oc]; k < L.lc_ptr[oc + 1]; ++k) {
+                    ci.push_back(qinv[L.lc_idx[k]]);
+                    cv.push_back(L.lc_val[k]);
+                }
+                cp[c + 1] = (int)ci.size();
+            }
+            D.lb_ptr = track(dev_upload(bp));
+            D.lb_idx = track(dev_upload(bi));
+            D.lb_val = track(dev_upload(bv));
             std::vector<int> tp, ti;
             std::vector<T> tv;
-            transpose(L.nB, L.nB, L.lb_ptr, L.lb_idx, L.lb_val, tp, ti, tv);
+            transpose(nB, nB, bp, bi, bv, tp, ti, tv);
             D.lt_ptr = track(dev_upload(tp));
             D.lt_idx = track(dev_upload(ti));
             D.lt_val = track(dev_upload(tv));
-            std::vector<int> rows;
-            schedule(L.nB, L.lb_ptr, L.lb_idx, true, D.fwd_ptr, rows);
-            D.fwd_rows = track(dev_upload(rows));
-            schedule(L.nB, tp, ti, false, D.bwd_ptr, rows);
-            D.bwd_rows = track(dev_upload(rows));
-            D.lc_ptr = track(dev_upload(L.lc_ptr));
-            D.lc_idx = track(dev_upload(L.lc_idx));
-            D.lc_val = track(dev_upload(L.lc_val));
-            transpose(D.nC, L.nB, L.lc_ptr, L.lc_idx, L.lc_val, tp, ti, tv);
-            D.lct_ptr = track(dev_upload(tp));
-            D.lct_idx = track(dev_upload(ti));
-            D.lct_val = track(dev_upload(tv));
-            D.buf = track(dev_alloc<T>(L.n));
-            D.out = track(dev_alloc<T>(L.n));
-            D.t1 = track(dev_alloc<T>(std::max(L.nB, 1)));
-            D.t2 = track(dev_alloc<T>(std::max(L.nB, 1)));
+            D.sf = upload_sell(build_sell(nB, bp, bi, bv));
+            D.sb = upload_sell(build_sell(nB, tp, ti, tv));
+            // dependency levels from the triangular (factor) numbering
+            {
+                std::vector<int> rows, otp, oti;
+                std::vector<T> otv;
+                schedule(nB, L.lb_ptr, L.lb_idx, true, D.fwd_ptr, rows);
+                for (int &r : rows) r = qinv[r];
+                D.fwd_rows = track(dev_upload(rows));
+                transpose(nB, nB, L.lb_ptr, L.lb_idx, L.lb_val, otp, oti, otv);
+                schedule(nB, otp, oti, false, D.bwd_ptr, rows);
+                for (int &r : rows) r = qinv[r];
+                D.bwd_rows = track(dev_upload(rows));
+            }
+            D.sc = upload_sell(build_sell(nC, cp, ci, cv));
+            transpose(nC, nB, cp, ci, cv, tp, ti, tv);
+            D.sct = upload_sell(build_sell(nB, tp, ti, tv));
+            D.buf = track(dev_alloc<T>(n));
+            D.out = track(dev_alloc<T>(n));
+            D.t1 = track(dev_alloc<T>(std::max(nB, 1)));
+            D.t2 = track(dev_alloc<T>(std::max(nB, 1)));
             D.nnz_lb = (long)L.lb_idx.size();
             D.nnz_lc = (long)L.lc_idx.size();
             res.fwd_levels.push_back((int)D.fwd_ptr.size() - 1);
             res.bwd_levels.push_back((int)D.bwd_ptr.size() - 1);
             lv_.push_back(std::move(D));
+            // relabelling of the next level's input (this level's C part)
+            std::vector<long> norig(nC);
+            for (int c = 0; c < nC; ++c) norig[c] = orig[permN[nB + c]];
+            orig.swap(norig);
+            cnew.assign(nC, 0);
+            qC_last.assign(nC, 0);
+            for (int c = 0; c < nC; ++c) {
+                cnew[q[nB + c] - nB] = c;
+                qC_last[c] = q[nB + c] - nB;
+            }
         }
         if (f_.m > 0) {
-            std::vector<T> mv(f_.dinv.begin(), f_.dinv.end());
+            const int m = f_.m;
+            std::vector<T> mv((size_t)m * m);
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) {
+                    const int oa = nlev ? qC_last[a] : a, ob = nlev ? qC_last[b] : b;
+                    mv[(size_t)a * m + b] = (T)f_.dinv[(size_t)oa * m + ob];
+                }
             dmat_ = track(dev_upload(mv));
-            dx_ = track(dev_alloc<T>(f_.m));
-            dy_ = track(dev_alloc<T>(f_.m));
+            dx_ = track(dev_alloc<T>(m));
+            dy_ = track(dev_alloc<T>(m));
         }
         JDHIP_CHECK(hipDeviceSynchronize());
     }
@@ -176,14 +307,36 @@ class IldlImpl : public IldlImplBase {
                 ildl_trsv_level<T>(s, rows + lp[l], lp[l + 1] - lp[l], ptr, idx, val, D.buf);
             return;
         }
-        // Jacobi sweeps x <- b - N x, b = D.buf
-        JDHIP_CHECK(hipMemcpyAsync(D.t1, D.buf, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
-        T *xin = D.t1, *xout = D.t2;
+        // Jacobi sweeps x <- b - N x from x = b (b = D.buf).  The first sweep
+        // reads b itself; the last writes into D.buf, which is safe because
+        // every row reads only its own b entry and all x entries come from
+        // the other buffer (needs at least 2 sweeps; 1 sweep goes via t1).
+        const SellDev<T> &M = fwd ? D.sf : D.sb;
+        const T *xin = D.buf;
+        T *bufs[2] = {D.t1, D.t2};
         for (int k = 0; k < sweeps_; ++k) {
-            ildl_jacobi_sweep<T>(s, D.nB, ptr, idx, val, D.buf, xin, xout);
-            std::swap(xin, xout);
+            const bool last = (k == sweeps_ - 1) && sweeps_ >= 2;
+            T *xout = last ? D.buf : bufs[k & 1];
+            ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, D.buf,
+                                      xin, xout);
+            xin = xout;
         }
-        JDHIP_CHECK(hipMemcpyAsync(D.buf, xin, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
+        if (sweeps_ == 1)
+            JDHIP_CHECK(hipMemcpyAsync(D.buf, D.t1, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
+        (void)ptr;
+        (void)idx;
+        (void)val;
+    }
+
+    SellDev<T> upload_sell(const SellHost<T> &h) {
+        SellDev<T> d;
+        d.nslices = (int)h.wid.size();
+        d.vptr = track(dev_upload(h.vptr));
+        d.wid = track(dev_upload(h.wid));
+        d.rowid = track(dev_upload(h.rowid));
+        d.col = track(dev_upload(h.col));
+        d.val = track(dev_upload(h.val));
+        return d;
     }
 
     int sweeps() const override { return sweeps_; }
@@ -267,7 +420,8 @@ class IldlImpl : public IldlImplBase {
             }
             tri_solve(D, true);
             if (D.nC > 0)
-                ildl_spmv_sub<T>(s, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, D.buf, D.buf + D.nB);
+                ildl_sub_sell<T>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
+                                 D.sc.col, D.buf, D.buf + D.nB);
             ildl_dscale<T>(s, D.nB, D.dinv, D.buf);
         }
         const T *zc = nullptr;
@@ -279,7 +433,8 @@ class IldlImpl : public IldlImplBase {
         for (int l = L - 1; l >= 0; --l) {
             DevLevel<T> &D = lv_[l];
             if (D.nC > 0)
-                ildl_spmv_sub<T>(s, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, zc, D.buf);
+                ildl_sub_sell<T>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
+                                 D.sct.val, D.sct.col, zc, D.buf);
             tri_solve(D, false);
             if (l == 0) ildl_scatter<T, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
             else ildl_scatter<T, T>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
@@ -313,12 +468,14 @@ class IldlImpl : public IldlImplBase {
         for (size_t i = 0; i < f_.dinv.size(); ++i) minv[i] = (double)(T)f_.dinv[i];
     }
     long apply_bytes() const override {
-        // per level: 2 sweeps over L_B (value + index), L_CB and L_CB^T once,
+        // per level: one pass over L_B per sweep and direction (values,
+        // indices, row map, b, x, result), L_CB and L_CB^T once,
         // gather/scatter (perm + scal + in/out), D^-1, dense inverse.
         const long s = sizeof(T);
+        const long k = sweeps_ > 0 ? sweeps_ : 1;
         long b = 0;
         for (const auto &D : lv_) {
-            b += 2 * ((s + 4) * D.nnz_lb + 4L * (D.nB + 1) + 2 * s * D.nB);
+            b += 2 * k * ((s + 4) * D.nnz_lb + 4L * D.nB + 3 * s * D.nB);
             b += 2 * ((s + 4) * D.nnz_lc + 4L * (D.nC + D.nB + 2) + s * (D.nB + D.nC));
             b += 2 * (4L + s) * D.n + 2 * s * D.n + 2 * s * D.nB;
         }
@@ -326,6 +483,35 @@ class IldlImpl : public IldlImplBase {
         return b;
     }
     bool single() const override { return sizeof(T) == 4; }
+
+    int time_sweep(int reps, double *avg_ms, long *alg_bytes) override {
+        if (lv_.empty() || reps <= 0) return -1;
+        const DevLevel<T> &D = lv_[0];
+        hipStream_t s = ctx_->stream;
+        JDHIP_CHECK(hipMemsetAsync(D.buf, 0, sizeof(T) * D.n, s));
+        JDHIP_CHECK(hipMemsetAsync(D.t1, 0, sizeof(T) * std::max(D.nB, 1), s));
+        hipEvent_t e0, e1;
+        JDHIP_CHECK(hipEventCreate(&e0));
+        JDHIP_CHECK(hipEventCreate(&e1));
+        const SellDev<T> &M = D.sf;
+        auto one = [&]() {
+            ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, D.buf,
+                                      D.t1, D.t2);
+        };
+        one();
+        JDHIP_CHECK(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) one();
+        JDHIP_CHECK(hipEventRecord(e1, s));
+        JDHIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        *avg_ms = ms / reps;
+        const long sz = sizeof(T);
+        *alg_bytes = (sz + 4) * D.nnz_lb + 4L * D.nB + 3 * sz * D.nB;
+        return 0;
+    }
 
   private:
     template <typename X>

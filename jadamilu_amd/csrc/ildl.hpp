@@ -51,6 +51,9 @@ class IldlImplBase {
     // level-scheduled solve and k Jacobi sweeps whose apply stays within
     // relative distance tol of the exact apply on a fixed random vector.
     virtual int tune_sweeps(double tol) = 0;
+    // Time reps launches of one Jacobi sweep over the first level's L_B (the
+    // largest single kernel of the apply) on the context stream.
+    virtual int time_sweep(int reps, double *avg_ms, long *alg_bytes) = 0;
     virtual int sweeps() const = 0;
 };
 

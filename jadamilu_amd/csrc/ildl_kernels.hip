@@ -74,6 +74,60 @@ __global__ __launch_bounds__(kB) void k_jacobi(int nrows, const int *__restrict_
 }
 
 template <typename T>
+__global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__restrict__ vptr,
+                                                    const int *__restrict__ wid,
+                                                    const int *__restrict__ rowid,
+                                                    const T *__restrict__ val,
+                                                    const int *__restrict__ col,
+                                                    const T *b, const T *xin, T *xout) {
+    // b and xout may be the same array (the last sweep writes in place: each
+    // lane reads b only at its own row before writing that row)
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (long sl = (long)blockIdx.x * (kB / 64) + wv; sl < nslices;
+         sl += (long)gridDim.x * (kB / 64)) {
+        const int r = rowid[sl * 64 + lane];
+        if (r < 0) continue;
+        const int w = wid[sl];
+        const T *v = val + vptr[sl] + lane;
+        const int *c = col + vptr[sl] + lane;
+        T s0 = T(0), s1 = T(0);
+        int k = 0;
+        for (; k + 1 < w; k += 2) {
+            s0 += v[k * 64] * xin[c[k * 64]];
+            s1 += v[(k + 1) * 64] * xin[c[(k + 1) * 64]];
+        }
+        if (k < w) s0 += v[k * 64] * xin[c[k * 64]];
+        xout[r] = b[r] - (s0 + s1);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__restrict__ vptr,
+                                                 const int *__restrict__ wid,
+                                                 const int *__restrict__ rowid,
+                                                 const T *__restrict__ val,
+                                                 const int *__restrict__ col,
+                                                 const T *__restrict__ x, T *__restrict__ y) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (long sl = (long)blockIdx.x * (kB / 64) + wv; sl < nslices;
+         sl += (long)gridDim.x * (kB / 64)) {
+        const int r = rowid[sl * 64 + lane];
+        if (r < 0) continue;
+        const int w = wid[sl];
+        const T *v = val + vptr[sl] + lane;
+        const int *c = col + vptr[sl] + lane;
+        T s0 = T(0), s1 = T(0);
+        int k = 0;
+        for (; k + 1 < w; k += 2) {
+            s0 += v[k * 64] * x[c[k * 64]];
+            s1 += v[(k + 1) * 64] * x[c[(k + 1) * 64]];
+        }
+        if (k < w) s0 += v[k * 64] * x[c[k * 64]];
+        y[r] -= s0 + s1;
+    }
+}
+
+template <typename T>
 __global__ __launch_bounds__(kB) void k_spmv_sub(int nrows, const int *__restrict__ ptr,
                                                  const int *__restrict__ idx,
                                                  const T *__restrict__ val,
@@ -134,6 +188,21 @@ void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx,
     k_jacobi<T><<<blocks_for(nrows, kB / kLpr), kB, 0, s>>>(nrows, ptr, idx, val, b, xin, xout);
 }
 template <typename T>
+void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
+                            const int *rowid, const T *val, const int *col, const T *b,
+                            const T *xin, T *xout) {
+    if (nslices <= 0) return;
+    k_jacobi_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
+        nslices, vptr, wid, rowid, val, col, b, xin, xout);
+}
+template <typename T>
+void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
+                   const int *rowid, const T *val, const int *col, const T *x, T *y) {
+    if (nslices <= 0) return;
+    k_sub_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(nslices, vptr, wid, rowid,
+                                                                      val, col, x, y);
+}
+template <typename T>
 void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
                    const T *x, T *y) {
     if (nrows <= 0) return;
@@ -155,6 +224,11 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
                                      const T *, T *);                                        \
     template void ildl_jacobi_sweep<T>(hipStream_t, int, const int *, const int *, const T *, \
                                        const T *, const T *, T *);                           \
+    template void ildl_jacobi_sweep_sell<T>(hipStream_t, int, const long *, const int *,      \
+                                            const int *, const T *, const int *, const T *,  \
+                                            const T *, T *);                                 \
+    template void ildl_sub_sell<T>(hipStream_t, int, const long *, const int *, const int *,  \
+                                   const T *, const int *, const T *, T *);                  \
     template void ildl_spmv_sub<T>(hipStream_t, int, const int *, const int *, const T *,     \
                                    const T *, T *);                                          \
     template void ildl_dscale<T>(hipStream_t, int, const T *, T *);                          \

@@ -13,7 +13,7 @@ namespace jdhip {
 
 constexpr int kBlock = 256;      // threads per block (4 wave64)
 constexpr int kMaxCols = 64;     // max columns of a tall-skinny product
-constexpr int kMaxRedBlocks = 1024;
+constexpr int kMaxRedBlocks = 4096;  // enough waves in flight to cover HBM latency
 
 // Device-resident scalars of the inner solver (one per engine).
 struct InnerScalars {

@@ -213,6 +213,15 @@ class Context:
         if rc != 0:
             raise RuntimeError(self.error())
 
+    def ildl_time_sweep(self, reps=20):
+        """(avg ms, algorithmic bytes) of one Jacobi sweep launch over the
+        first level's L_B, timed with HIP events on the context stream."""
+        ms, nb = C.c_double(), C.c_long()
+        rc = self._L.jdhip_ildl_time_sweep(self.h, reps, C.byref(ms), C.byref(nb))
+        if rc != 0:
+            raise RuntimeError(self.error() or "no ILDL preconditioner")
+        return ms.value, nb.value
+
     def ildl_device_info(self):
         f, b, nb = C.c_int(), C.c_int(), C.c_long()
         if self._L.jdhip_ildl_device_info(self.h, C.byref(f), C.byref(b), C.byref(nb)) != 0:
