@@ -112,6 +112,29 @@ jdhip_ctx *jdhip_legacy_context(void);
  * with their own kernels should use it (or synchronise with it). */
 void *jdhip_stream(jdhip_ctx *ctx);
 
+/* ---- batches of independent problems ------------------------------------ */
+/* One problem of a batch: a 1-based upper CSR on the host and the DPJD
+ * parameters (see jdhip_pjd); outputs are written into the item. */
+typedef struct {
+    int n;
+    const int *ia, *ja;
+    const double *a;
+    int neig, isearch, madspace, maxiter;
+    double sigma, tol, shift, droptol, mem;
+    int icntl[5];
+    double *eigs, *res; /* host, neig entries each */
+    double *x;          /* host n*neig eigenvectors (column-major), or NULL */
+    int info, iter, neig_out;
+    double gap;
+} jdhip_batch_item;
+/* Solve every item on HIP device `device` with nworkers worker threads, each
+ * owning a context and a HIP stream; items are taken largest first.  Each
+ * item starts from the state of a fresh process, so its results equal those
+ * of a single jdhip_pjd call.  No reference counterpart (the reference solves
+ * one problem per call).  Returns the number of items with INFO != 0, or -1
+ * on invalid arguments. */
+int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers);
+
 /* ---- symmetric SpMV (MATVECA, JDaux.F:3397-3418) ------------------------- */
 /* Upload a 1-based upper-triangle CSR (diagonal anywhere in each row) as the
  * device operator of ctx; 0 on success.  lanes_per_row: 0 = autotune (the

@@ -36,6 +36,17 @@ class JdhipStats(C.Structure):
                 ("precond_negpiv", C.c_int), ("precond_sweeps", C.c_int)]
 
 
+class JdhipBatchItem(C.Structure):
+    """Mirror of jdhip_batch_item (include/jadamilu_amd.h)."""
+    _fields_ = [("n", C.c_int), ("ia", _i), ("ja", _i), ("a", _d),
+                ("neig", C.c_int), ("isearch", C.c_int), ("madspace", C.c_int),
+                ("maxiter", C.c_int), ("sigma", C.c_double), ("tol", C.c_double),
+                ("shift", C.c_double), ("droptol", C.c_double), ("mem", C.c_double),
+                ("icntl", C.c_int * 5), ("eigs", _d), ("res", _d), ("x", _d),
+                ("info", C.c_int), ("iter", C.c_int), ("neig_out", C.c_int),
+                ("gap", C.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "dpjd_": (None, [_i, _d, _i, _i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _d, _d, _d, _d,
@@ -80,6 +91,7 @@ SIGNATURES = {
     "jdhip_adjoint_csr": (C.c_int, [_v, C.c_int, _v, _v, _v, C.c_long, C.c_int, _d, _v]),
     "jdhip_adjoint_diag": (C.c_int, [_v, C.c_int, _v, C.c_long, C.c_int, _d, _v]),
     "jdhip_last_error": (C.c_char_p, [_v]),
+    "jdhip_pjd_batch": (C.c_int, [C.c_int, C.c_int, C.POINTER(JdhipBatchItem), C.c_int]),
 }
 
 

@@ -147,6 +147,52 @@ def pjd_host(n, a, ja, ia, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20, 
                     info=INFO.value, gap=GAP.value, shift=SH.value, droptol=DT.value)
 
 
+def pjd_batch(problems, *, device=0, workers=8, want_x=False):
+    """jdhip_pjd_batch: solve independent problems on one device.
+
+    problems: list of dicts with keys ia, ja, a (1-based upper CSR) and
+    optionally neig (5), isearch (0), sigma (0.0), madspace (20), maxiter
+    (1000), tol (1e-10), shift (0.0), droptol (1e-3), mem (20.0), icntl
+    ((0, 1, 0, 0, 0)).  Returns a list of JDResult (x only if want_x)."""
+    from . import JdhipBatchItem
+    L = lib()
+    items = (JdhipBatchItem * len(problems))()
+    keep = []
+    for k, pr in enumerate(problems):
+        ia = np.ascontiguousarray(pr["ia"], dtype=np.int32)
+        ja = np.ascontiguousarray(pr["ja"], dtype=np.int32)
+        a = np.ascontiguousarray(pr["a"], dtype=np.float64)
+        n = ia.size - 1
+        neig = int(pr.get("neig", 5))
+        eigs, res = np.zeros(neig), np.zeros(neig)
+        x = np.zeros(n * neig) if want_x else None
+        keep += [ia, ja, a, eigs, res, x]
+        it = items[k]
+        it.n, it.ia, it.ja, it.a = n, _p(ia, C.c_int), _p(ja, C.c_int), _p(a)
+        it.neig, it.isearch = neig, int(pr.get("isearch", 0))
+        it.madspace, it.maxiter = int(pr.get("madspace", 20)), int(pr.get("maxiter", 1000))
+        it.sigma, it.tol = float(pr.get("sigma", 0.0)), float(pr.get("tol", 1e-10))
+        it.shift, it.droptol = float(pr.get("shift", 0.0)), float(pr.get("droptol", 1e-3))
+        it.mem = float(pr.get("mem", 20.0))
+        for c, v in enumerate(pr.get("icntl", (0, 1, 0, 0, 0))):
+            it.icntl[c] = int(v)
+        it.eigs, it.res = _p(eigs), _p(res)
+        it.x = _p(x) if want_x else C.cast(None, C.POINTER(C.c_double))
+    rc = L.jdhip_pjd_batch(device, len(problems), items, workers)
+    if rc < 0:
+        raise ValueError("jdhip_pjd_batch: invalid arguments")
+    out = []
+    for k in range(len(problems)):
+        it = items[k]
+        n, ne = it.n, it.neig_out
+        eigs, res, x = keep[6 * k + 3], keep[6 * k + 4], keep[6 * k + 5]
+        out.append(JDResult(eigs=eigs[:ne].copy(), res=res[:ne].copy(),
+                            x=None if x is None else x[: n * ne].reshape(ne, n).T.copy(),
+                            neig=ne, iter=it.iter, info=it.info, gap=it.gap, shift=0.0,
+                            droptol=0.0))
+    return out
+
+
 def cleanup():
     lib().dpjdcleanup_()
 
