@@ -33,6 +33,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# progress lines of the host factorisation on stderr (it can run for minutes)
+os.environ.setdefault("JDHIP_VERBOSE", "1")
 
 HBM_PEAK_GBS = 8000.0
 

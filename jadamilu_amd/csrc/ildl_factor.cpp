@@ -4,7 +4,10 @@
 #include "ordering.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <numeric>
 #include <stdexcept>
@@ -150,6 +153,16 @@ static std::vector<int> rcm_order(const SymCsr &A) {
 
 namespace {
 
+// Progress lines on stderr when JDHIP_VERBOSE is set (long host
+// factorisations would otherwise run silently for minutes).
+bool verbose() {
+    const char *v = std::getenv("JDHIP_VERBOSE");
+    return v && *v && *v != '0';
+}
+double since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 template <typename T>
 struct LevelOut {
     IldlLevel<T> L;
@@ -175,6 +188,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         s[i] = (m > 0.0) ? pow2_near(1.0 / std::sqrt(m)) : 1.0;
     }
     // ---- ordering ----
+    const auto t_start = std::chrono::steady_clock::now();
     std::vector<int> order;
     if (prm.ordering == 2) order = amd_order(n, A.ptr.data(), A.idx.data());
     else if (prm.ordering == 1) order = rcm_order(A);
@@ -205,6 +219,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             }
         }
     }
+    const double t_order = since(t_start);
     // ---- Crout loop ----
     std::vector<int8_t> status(n, 0);  // 0 pending, 1 accepted, 2 deferred
     std::vector<int> bpos(n, -1), cpos(n, -1);
@@ -340,6 +355,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     out.maxcond = maxcond;
     out.nC = nC;
     L.nB = nB;
+    const double t_crout = since(t_start) - t_order;
     // ---- level output ----
     L.perm.assign(n, 0);
     for (int k = 0; k < n; ++k) {
@@ -493,6 +509,12 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         }
         S = std::move(U);
     }
+    if (verbose())
+        std::fprintf(stderr,
+                     "jdhip ildl: level n=%d nB=%d nC=%d condest=%g  ordering %.2fs  "
+                     "crout %.2fs  schur %.2fs\n",
+                     n, L.nB, out.nC, condest, t_order, t_crout,
+                     since(t_start) - t_order - t_crout);
 }
 
 // Dense LDL^T with Bunch-Kaufman pivoting (alpha = (1+sqrt(17))/8) of the
