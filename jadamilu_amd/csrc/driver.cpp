@@ -57,6 +57,7 @@ struct Solver {
     int factdgl = 0, factspd = 0, prvdr = 0;
     // build options of the multilevel ILDL
     int ildl_sweeps = -1, ildl_ordering = 2, ildl_dense_max = 1500;
+    int ildl_storage = 0, ildl_xcd = 0;  // device layout of the factor (measured best)
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -494,6 +495,8 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
             }
         }
         ildl = std::make_unique<IldlPrecond>(&ctx);
+        ildl->storage_order = ildl_storage;
+        ildl->xcd_remap = ildl_xcd;
         ildl->attach(h, ildl_sweeps);
         ildl_info = ildl->result;
         totaltime =
@@ -933,6 +936,8 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_dense_max") S.ildl_dense_max = (int)value;
     else if (k == "spmv_lanes") S.A.lpr = (int)value;
     else if (k == "inner_fp32") S.inner_fp32 = value != 0.0;
+    else if (k == "ildl_storage") S.ildl_storage = (int)value;
+    else if (k == "ildl_xcd") S.ildl_xcd = (int)value;
     else return -2;
     return 0;
 }
@@ -1078,6 +1083,8 @@ int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
         Solver &S = *ctx->S;
         S.release_precond();
         S.ildl = std::make_unique<jdhip::IldlPrecond>(&S.ctx);
+        S.ildl->storage_order = S.ildl_storage;
+        S.ildl->xcd_remap = S.ildl_xcd;
         S.ildl->attach(f->h, sweeps);
         S.factdgl = -1;
         S.n = f->h.single ? (f->h.ff.levels.empty() ? f->h.ff.m : f->h.ff.levels[0].n)

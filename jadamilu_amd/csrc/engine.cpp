@@ -23,7 +23,7 @@ void DevCtx::init(int dev) {
     device = dev;
     JDHIP_CHECK(hipSetDevice(dev));
     JDHIP_CHECK(hipStreamCreate(&stream));
-    JDHIP_CHECK(hipMalloc(&rb.partials, sizeof(double) * kMaxRedBlocks * kMaxCols));
+    JDHIP_CHECK(hipMalloc(&rb.partials, sizeof(double) * kMaxPartials));
     JDHIP_CHECK(hipMalloc(&rb.counter, sizeof(unsigned) * 16));
     JDHIP_CHECK(hipMemset(rb.counter, 0, sizeof(unsigned) * 16));
     JDHIP_CHECK(hipMalloc(&d_small, sizeof(double) * 8192));

@@ -163,8 +163,8 @@ struct DevLevel {
 template <typename T>
 class IldlImpl : public IldlImplBase {
   public:
-    IldlImpl(DevCtx *ctx, IldlMultilevel<T> &&f, int sweeps)
-        : ctx_(ctx), f_(std::move(f)), sweeps_(sweeps) {}
+    IldlImpl(DevCtx *ctx, IldlMultilevel<T> &&f, int sweeps, int storage_order, int xcd)
+        : ctx_(ctx), f_(std::move(f)), sweeps_(sweeps), storage_(storage_order), xcd_(xcd) {}
     ~IldlImpl() override {
         hipStreamSynchronize(ctx_->stream);
         for (void *p : allocs_) hipFree(p);
@@ -195,9 +195,11 @@ class IldlImpl : public IldlImplBase {
             for (int j = 0; j < n; ++j) scal[cnew.empty() ? j : cnew[j]] = L.scal[j];
             std::vector<int> q(n), qinv(n);
             for (int i = 0; i < n; ++i) q[i] = i;
-            auto by_orig = [&](int a, int b) { return orig[perm[a]] < orig[perm[b]]; };
-            std::stable_sort(q.begin(), q.begin() + nB, by_orig);
-            std::stable_sort(q.begin() + nB, q.end(), by_orig);
+            if (storage_ == 1) {
+                auto by_orig = [&](int a, int b) { return orig[perm[a]] < orig[perm[b]]; };
+                std::stable_sort(q.begin(), q.begin() + nB, by_orig);
+                std::stable_sort(q.begin() + nB, q.end(), by_orig);
+            }
             for (int t = 0; t < n; ++t) qinv[q[t]] = t;
 
             DevLevel<T> D;
@@ -318,7 +320,7 @@ class IldlImpl : public IldlImplBase {
             const bool last = (k == sweeps_ - 1) && sweeps_ >= 2;
             T *xout = last ? D.buf : bufs[k & 1];
             ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, D.buf,
-                                      xin, xout);
+                                      xin, xout, xcd_);
             xin = xout;
         }
         if (sweeps_ == 1)
@@ -421,7 +423,7 @@ class IldlImpl : public IldlImplBase {
             tri_solve(D, true);
             if (D.nC > 0)
                 ildl_sub_sell<T>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
-                                 D.sc.col, D.buf, D.buf + D.nB);
+                                 D.sc.col, D.buf, D.buf + D.nB, xcd_);
             ildl_dscale<T>(s, D.nB, D.dinv, D.buf);
         }
         const T *zc = nullptr;
@@ -434,7 +436,7 @@ class IldlImpl : public IldlImplBase {
             DevLevel<T> &D = lv_[l];
             if (D.nC > 0)
                 ildl_sub_sell<T>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
-                                 D.sct.val, D.sct.col, zc, D.buf);
+                                 D.sct.val, D.sct.col, zc, D.buf, xcd_);
             tri_solve(D, false);
             if (l == 0) ildl_scatter<T, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
             else ildl_scatter<T, T>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
@@ -496,7 +498,7 @@ class IldlImpl : public IldlImplBase {
         const SellDev<T> &M = D.sf;
         auto one = [&]() {
             ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, D.buf,
-                                      D.t1, D.t2);
+                                      D.t1, D.t2, xcd_);
         };
         one();
         JDHIP_CHECK(hipEventRecord(e0, s));
@@ -537,6 +539,8 @@ class IldlImpl : public IldlImplBase {
     DevCtx *ctx_;
     IldlMultilevel<T> f_;
     int sweeps_;
+    int storage_;  // 0: factor order, 1: original-index order per level
+    int xcd_;      // XCD-contiguous block order in the SELL kernels
     std::vector<DevLevel<T>> lv_;
     std::vector<void *> allocs_;
     T *dmat_ = nullptr, *dx_ = nullptr, *dy_ = nullptr;
@@ -592,12 +596,12 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     const int k0 = sweeps < 0 ? 0 : sweeps;
     if (h.single) {
         IldlMultilevel<float> f = h.ff;
-        auto *p = new IldlImpl<float>(ctx_, std::move(f), k0);
+        auto *p = new IldlImpl<float>(ctx_, std::move(f), k0, storage_order, xcd_remap);
         impl_.reset(p);
         p->upload(result);
     } else {
         IldlMultilevel<double> f = h.fd;
-        auto *p = new IldlImpl<double>(ctx_, std::move(f), k0);
+        auto *p = new IldlImpl<double>(ctx_, std::move(f), k0, storage_order, xcd_remap);
         impl_.reset(p);
         p->upload(result);
     }

@@ -82,6 +82,10 @@ class IldlPrecond {
     IldlImplBase *impl() { return impl_.get(); }
     bool ready() const { return (bool)impl_; }
     IldlResult result;
+    // device layout: storage order of each level (0 factor order, 1 original
+    // matrix index order) and XCD-contiguous block order in the sweeps
+    int storage_order = 0;
+    int xcd_remap = 0;
 
   private:
     DevCtx *ctx_;

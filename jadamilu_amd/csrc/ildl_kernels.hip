@@ -12,6 +12,18 @@ namespace jdhip {
 static constexpr int kB = 256;
 static constexpr int kLpr = 4;
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8), each with its own L2.  With xcd != 0 the kernels below use the
+// block id remapped so that every XCD works on one contiguous range of
+// slices: neighbouring rows share x entries, and the shared cache lines then
+// stay in one L2 instead of being fetched by all eight.
+__device__ __forceinline__ long logical_block(int xcd) {
+    if (!xcd) return blockIdx.x;
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned per = nb / 8, rem = nb % 8, x = b % 8, i = b / 8;
+    return (long)x * per + (x < rem ? x : rem) + i;
+}
+
 static inline int blocks_for(long n, int per_block, int cap = 8192) {
     long b = (n + per_block - 1) / per_block;
     return (int)std::max<long>(1, std::min<long>(b, cap));
@@ -79,11 +91,12 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
                                                     const int *__restrict__ rowid,
                                                     const T *__restrict__ val,
                                                     const int *__restrict__ col,
-                                                    const T *b, const T *xin, T *xout) {
+                                                    const T *b, const T *xin, T *xout,
+                                                    int xcd) {
     // b and xout may be the same array (the last sweep writes in place: each
     // lane reads b only at its own row before writing that row)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (long sl = (long)blockIdx.x * (kB / 64) + wv; sl < nslices;
+    for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < nslices;
          sl += (long)gridDim.x * (kB / 64)) {
         const int r = rowid[sl * 64 + lane];
         if (r < 0) continue;
@@ -107,9 +120,10 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
                                                  const int *__restrict__ rowid,
                                                  const T *__restrict__ val,
                                                  const int *__restrict__ col,
-                                                 const T *__restrict__ x, T *__restrict__ y) {
+                                                 const T *__restrict__ x, T *__restrict__ y,
+                                                 int xcd) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (long sl = (long)blockIdx.x * (kB / 64) + wv; sl < nslices;
+    for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < nslices;
          sl += (long)gridDim.x * (kB / 64)) {
         const int r = rowid[sl * 64 + lane];
         if (r < 0) continue;
@@ -190,17 +204,18 @@ void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx,
 template <typename T>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                             const int *rowid, const T *val, const int *col, const T *b,
-                            const T *xin, T *xout) {
+                            const T *xin, T *xout, int xcd) {
     if (nslices <= 0) return;
     k_jacobi_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
-        nslices, vptr, wid, rowid, val, col, b, xin, xout);
+        nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd);
 }
 template <typename T>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const T *x, T *y) {
+                   const int *rowid, const T *val, const int *col, const T *x, T *y,
+                   int xcd) {
     if (nslices <= 0) return;
     k_sub_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(nslices, vptr, wid, rowid,
-                                                                      val, col, x, y);
+                                                                      val, col, x, y, xcd);
 }
 template <typename T>
 void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
@@ -226,9 +241,9 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
                                        const T *, const T *, T *);                           \
     template void ildl_jacobi_sweep_sell<T>(hipStream_t, int, const long *, const int *,      \
                                             const int *, const T *, const int *, const T *,  \
-                                            const T *, T *);                                 \
+                                            const T *, T *, int);                            \
     template void ildl_sub_sell<T>(hipStream_t, int, const long *, const int *, const int *,  \
-                                   const T *, const int *, const T *, T *);                  \
+                                   const T *, const int *, const T *, T *, int);             \
     template void ildl_spmv_sub<T>(hipStream_t, int, const int *, const int *, const T *,     \
                                    const T *, T *);                                          \
     template void ildl_dscale<T>(hipStream_t, int, const T *, T *);                          \

@@ -29,15 +29,16 @@ void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx,
 // (one wave64, one lane per row) taken from windows of rows sorted by
 // length, entries of a slice stored column-major (value and column at
 // vptr[s] + k*64 + lane, k < wid[s]); rowid[s*64 + lane] is the row a lane
-// owns (-1 for padding lanes).
+// owns (-1 for padding lanes).  xcd != 0: XCD-contiguous block order.
 template <typename T>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                             const int *rowid, const T *val, const int *col, const T *b,
-                            const T *xin, T *xout);
+                            const T *xin, T *xout, int xcd);
 // y[r] -= sum_j M(r, j) x[j] over the rows of a SELL-C-sigma matrix
 template <typename T>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const T *x, T *y);
+                   const int *rowid, const T *val, const int *col, const T *x, T *y,
+                   int xcd);
 // y[r] -= sum_j M(r, j) x[j] for r < nrows, then optional y[r] *= dscale[r]
 template <typename T>
 void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,

@@ -14,6 +14,7 @@ namespace jdhip {
 constexpr int kBlock = 256;      // threads per block (4 wave64)
 constexpr int kMaxCols = 64;     // max columns of a tall-skinny product
 constexpr int kMaxRedBlocks = 4096;  // enough waves in flight to cover HBM latency
+constexpr int kMaxPartials = kMaxRedBlocks * 64;  // reduction scratch (doubles)
 
 // Device-resident scalars of the inner solver (one per engine).
 struct InnerScalars {
@@ -27,7 +28,7 @@ struct InnerScalars {
 
 // Reduction scratch owned by an engine.
 struct RedBuf {
-    double *partials;   // kMaxRedBlocks * kMaxCols
+    double *partials;   // kMaxPartials
     unsigned *counter;  // 8 counters
     double *out;        // kMaxCols results
 };

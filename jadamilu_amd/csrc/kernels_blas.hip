@@ -5,8 +5,8 @@
 //    ~8 blocks per CU so every kernel streams HBM with enough waves in flight.
 //  * Reductions are two-level and deterministic: xor-butterfly inside the
 //    wave (DPP/permute lowering), LDS across the 4 waves, per-block partials
-//    in global memory, and the last block to take a ticket from a device
-//    counter sums the partials in a fixed order.  The result is written to
+//    in global memory, then a one-block launch of the same kernel sums the
+//    partials in a fixed order (reduce.hpp).  The result is written to
 //    device memory so the next kernel on the stream consumes it directly.
 //  * The inner-solver steps (LSOLREVCOM, src/jd/JDaux.F:2989-3268) are fused
 //    so that every scalar (rho, alpha, ||r||, u'q, ||x||) is produced on the
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_gemv_n(double *__restrict__ w,
                                                    const double *__restrict__ V, long ldv,
                                                    int jv, const double *__restrict__ y,
                                                    double alpha, double beta, long n,
-                                                   double *partials, unsigned *counter,
+                                                   double *partials, int nparts,
                                                    double *nrm_out) {
     __shared__ double ys[NC];
     if ((int)threadIdx.x < jv) ys[threadIdx.x] = alpha * y[threadIdx.x];
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_gemv_n(double *__restrict__ w,
     }
     if (NORM) {
         __shared__ double fin[1];
-        if (grid_reduce<1>(acc, 1, partials, counter, fin))
+        if (grid_reduce<1>(acc, 1, partials, nparts, fin))
             if (threadIdx.x == 0) nrm_out[0] = fin[0];
     }
 }
@@ -80,12 +80,16 @@ template <int NC>
 static void gemv_n_t(hipStream_t s, const RedBuf &rb, double *w, const double *V, long ldv,
                      int jv, const double *y, double alpha, double beta, long n,
                      double *nrm) {
-    if (nrm)
-        k_gemv_n<NC, true><<<grid_for(n), kBlock, 0, s>>>(w, V, ldv, jv, y, alpha, beta, n,
-                                                          rb.partials, rb.counter + 1, nrm);
-    else
+    if (nrm) {
+        const int g = grid_for(n);
+        k_gemv_n<NC, true><<<g, kBlock, 0, s>>>(w, V, ldv, jv, y, alpha, beta, n, rb.partials,
+                                                0, nrm);
+        k_gemv_n<NC, true><<<1, kBlock, 0, s>>>(w, V, ldv, jv, y, alpha, beta, 0, rb.partials,
+                                                g, nrm);
+    } else {
         k_gemv_n<NC, false><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(
-            w, V, ldv, jv, y, alpha, beta, n, rb.partials, rb.counter + 1, nrm);
+            w, V, ldv, jv, y, alpha, beta, n, rb.partials, 0, nrm);
+    }
 }
 
 void gemv_n(hipStream_t s, const RedBuf &rb, double *w, const double *V, long ldv, int jv,
@@ -101,7 +105,7 @@ void gemv_n(hipStream_t s, const RedBuf &rb, double *w, const double *V, long ld
 
 __global__ __launch_bounds__(kBlock) void k_axpy_nrm2(double *__restrict__ y, double a,
                                                       const double *__restrict__ x, long n,
-                                                      double *partials, unsigned *counter,
+                                                      double *partials, int nparts,
                                                       double *out) {
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
@@ -111,30 +115,34 @@ __global__ __launch_bounds__(kBlock) void k_axpy_nrm2(double *__restrict__ y, do
         acc[0] += yi * yi;
     }
     __shared__ double fin[1];
-    if (grid_reduce<1>(acc, 1, partials, counter, fin))
+    if (grid_reduce<1>(acc, 1, partials, nparts, fin))
         if (threadIdx.x == 0) out[0] = fin[0];
 }
 
 void axpy_nrm2(hipStream_t s, const RedBuf &rb, double *y, double a, const double *x, long n,
                double *out) {
-    k_axpy_nrm2<<<grid_for(n), kBlock, 0, s>>>(y, a, x, n, rb.partials, rb.counter + 2, out);
+    const int g = grid_for(n);
+    k_axpy_nrm2<<<g, kBlock, 0, s>>>(y, a, x, n, rb.partials, 0, out);
+    k_axpy_nrm2<<<1, kBlock, 0, s>>>(y, a, x, 0, rb.partials, g, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot(const double *__restrict__ x,
                                                 const double *__restrict__ y, long n,
-                                                double *partials, unsigned *counter,
+                                                double *partials, int nparts,
                                                 double *out) {
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
         acc[0] += x[i] * y[i];
     __shared__ double fin[1];
-    if (grid_reduce<1>(acc, 1, partials, counter, fin))
+    if (grid_reduce<1>(acc, 1, partials, nparts, fin))
         if (threadIdx.x == 0) out[0] = fin[0];
 }
 
 void dot(hipStream_t s, const RedBuf &rb, const double *x, const double *y, long n,
          double *out) {
-    k_dot<<<grid_for(n), kBlock, 0, s>>>(x, y, n, rb.partials, rb.counter + 2, out);
+    const int g = grid_for(n);
+    k_dot<<<g, kBlock, 0, s>>>(x, y, n, rb.partials, 0, out);
+    k_dot<<<1, kBlock, 0, s>>>(x, y, 0, rb.partials, g, out);
 }
 
 // out(:,j) = sum_k V(:,k) Q(k,j): each thread owns rows and keeps the K
@@ -220,7 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_argmax(const double *__restrict__ x,
 }
 
 void argmax_abs(hipStream_t s, const RedBuf &rb, const double *x, long n, double *out) {
-    k_argmax<<<grid_for(n), kBlock, 0, s>>>(x, n, rb.partials, rb.counter + 3, out);
+    // rare (once per restart): few blocks keep the ticket atomics cheap
+    k_argmax<<<grid_for(n, kBlock, 256), kBlock, 0, s>>>(x, n, rb.partials, rb.counter + 3, out);
 }
 
 }  // namespace jdhip

@@ -17,7 +17,7 @@ namespace jdhip {
 template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_multidot(const T *__restrict__ V, long ldv, int jv,
                                                      const T *__restrict__ w, long n,
-                                                     double *partials, unsigned *counter,
+                                                     double *partials, int nparts,
                                                      double *out) {
     double acc[NC];
 #pragma unroll
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const T *__restrict__ V, lo
             if (c < jv) acc[c] += (double)V[c * ldv + i] * wi;
     }
     __shared__ double fin[NC];
-    if (grid_reduce<NC>(acc, jv, partials, counter, fin)) {
+    if (grid_reduce<NC>(acc, jv, partials, nparts, fin)) {
         if ((int)threadIdx.x < jv) out[threadIdx.x] = fin[threadIdx.x];
     }
 }
@@ -39,14 +39,20 @@ void multidot(hipStream_t s, const RedBuf &rb, const T *V, long ldv, int jv, con
               double *out) {
     if (jv <= 0) return;
     const int g = grid_for(n);
-    unsigned *c = rb.counter;
-    if (jv <= 1) k_multidot<T, 1><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
-    else if (jv <= 2) k_multidot<T, 2><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
-    else if (jv <= 4) k_multidot<T, 4><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
-    else if (jv <= 8) k_multidot<T, 8><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
-    else if (jv <= 16) k_multidot<T, 16><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
-    else if (jv <= 32) k_multidot<T, 32><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
-    else k_multidot<T, 64><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, rb.partials, c, out);
+    double *P = rb.partials;
+#define JDHIP_MD(NC)                                                                    \
+    do {                                                                                \
+        k_multidot<T, NC><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, P, 0, out);            \
+        k_multidot<T, NC><<<1, kBlock, 0, s>>>(V, ldv, jv, w, 0, P, g, out);            \
+    } while (0)
+    if (jv <= 1) JDHIP_MD(1);
+    else if (jv <= 2) JDHIP_MD(2);
+    else if (jv <= 4) JDHIP_MD(4);
+    else if (jv <= 8) JDHIP_MD(8);
+    else if (jv <= 16) JDHIP_MD(16);
+    else if (jv <= 32) JDHIP_MD(32);
+    else JDHIP_MD(64);
+#undef JDHIP_MD
 }
 
 // ---------------------------------------------------------------------------
@@ -58,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
                                                  const T *__restrict__ val,
                                                  const T *__restrict__ x, T *__restrict__ y,
                                                  double eta_host, const T *__restrict__ u,
-                                                 double *partials, unsigned *counter,
+                                                 double *partials, int nparts,
                                                  double *udot_out) {
     constexpr int GPB = kBlock / LPR;
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
@@ -79,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
     }
     if (UDOT) {
         __shared__ double fin[1];
-        if (grid_reduce<1>(acc, 1, partials, counter, fin))
+        if (grid_reduce<1>(acc, 1, partials, nparts, fin))
             if (threadIdx.x == 0) udot_out[0] = fin[0];
     }
 }
@@ -90,23 +96,28 @@ static void spmv_lpr(hipStream_t s, const RedBuf &rb, int n, const int *rp, cons
     constexpr int GPB = kBlock / LPR;
     long rows_blocks = ((long)n + GPB - 1) / GPB;
     const bool has_eta = eta != 0.0;
-    unsigned *cnt = rb.counter + 4;
+    double *P = rb.partials;
     if (u) {
         int g = (int)std::min<long>(rows_blocks, kMaxRedBlocks);
-        if (has_eta)
-            k_spmv<T, LPR, true, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u,
-                                                           rb.partials, cnt, udot);
-        else
-            k_spmv<T, LPR, false, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u,
-                                                            rb.partials, cnt, udot);
+        if (has_eta) {
+            k_spmv<T, LPR, true, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+                                                           udot);
+            k_spmv<T, LPR, true, true><<<1, kBlock, 0, s>>>(0, rp, ci, val, x, y, eta, u, P, g,
+                                                           udot);
+        } else {
+            k_spmv<T, LPR, false, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+                                                            udot);
+            k_spmv<T, LPR, false, true><<<1, kBlock, 0, s>>>(0, rp, ci, val, x, y, eta, u, P, g,
+                                                            udot);
+        }
     } else {
         int g = (int)std::min<long>(rows_blocks, 8192);
         if (has_eta)
-            k_spmv<T, LPR, true, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u,
-                                                            rb.partials, cnt, udot);
+            k_spmv<T, LPR, true, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+                                                            udot);
         else
-            k_spmv<T, LPR, false, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u,
-                                                             rb.partials, cnt, udot);
+            k_spmv<T, LPR, false, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+                                                             udot);
     }
 }
 
@@ -116,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     int n, int nslices, const long *__restrict__ vptr, const long *__restrict__ iptr,
     const int *__restrict__ wm, const T *__restrict__ val, const char *__restrict__ idx,
     const T *__restrict__ x, T *__restrict__ y, double eta_host, const T *__restrict__ u,
-    double *partials, unsigned *counter, double *udot_out) {
+    double *partials, int nparts, double *udot_out) {
     constexpr int WPB = kBlock / 64;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const T eta = (T)eta_host;
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     }
     if (UDOT) {
         __shared__ double fin[1];
-        if (grid_reduce<1>(acc, 1, partials, counter, fin))
+        if (grid_reduce<1>(acc, 1, partials, nparts, fin))
             if (threadIdx.x == 0) udot_out[0] = fin[0];
     }
 }
@@ -163,27 +174,35 @@ static void spmv_sell(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T 
     constexpr int WPB = kBlock / 64;
     const long blocks = ((long)A.nslices + WPB - 1) / WPB;
     const bool has_eta = eta != 0.0;
-    unsigned *cnt = rb.counter + 4;
+    double *P = rb.partials;
     if (u) {
-        int g = (int)std::min<long>(blocks, kMaxRedBlocks);
-        if (has_eta)
+        // one partial sum per block: the full grid fits the scratch
+        int g = (int)std::min<long>(blocks, kMaxPartials);
+        if (has_eta) {
             k_spmv_sell<T, true, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr, A.s_iptr,
                                                           A.s_wm, val, A.s_idx, x, y, eta, u,
-                                                          rb.partials, cnt, udot);
-        else
+                                                          P, 0, udot);
+            k_spmv_sell<T, true, true><<<1, kBlock, 0, s>>>(A.n, 0, A.s_vptr, A.s_iptr,
+                                                          A.s_wm, val, A.s_idx, x, y, eta, u,
+                                                          P, g, udot);
+        } else {
             k_spmv_sell<T, false, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
                                                            A.s_iptr, A.s_wm, val, A.s_idx, x, y,
-                                                           eta, u, rb.partials, cnt, udot);
+                                                           eta, u, P, 0, udot);
+            k_spmv_sell<T, false, true><<<1, kBlock, 0, s>>>(A.n, 0, A.s_vptr, A.s_iptr,
+                                                           A.s_wm, val, A.s_idx, x, y, eta, u,
+                                                           P, g, udot);
+        }
     } else {
         int g = (int)std::min<long>(blocks, 1L << 20);
         if (has_eta)
             k_spmv_sell<T, true, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
                                                            A.s_iptr, A.s_wm, val, A.s_idx, x, y,
-                                                           eta, u, rb.partials, cnt, udot);
+                                                           eta, u, P, 0, udot);
         else
             k_spmv_sell<T, false, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
                                                             A.s_iptr, A.s_wm, val, A.s_idx, x,
-                                                            y, eta, u, rb.partials, cnt, udot);
+                                                            y, eta, u, P, 0, udot);
     }
 }
 
@@ -242,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_project_rho(T *__restrict__ z,
                                                            const T *__restrict__ r, long n,
                                                            InnerScalars *sc,
                                                            const double *__restrict__ ycoef,
-                                                           double *partials, unsigned *counter) {
+                                                           double *partials, int nparts) {
     if (sc->flag) return;
     __shared__ T ys[NC];
     if ((int)threadIdx.x < jq) ys[threadIdx.x] = (T)(-ycoef[threadIdx.x]);
@@ -257,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_project_rho(T *__restrict__ z,
         acc[0] += (double)r[i] * (double)zi;
     }
     __shared__ double fin[1];
-    if (grid_reduce<1>(acc, 1, partials, counter, fin)) {
+    if (grid_reduce<1>(acc, 1, partials, nparts, fin)) {
         if (threadIdx.x == 0) {
             double rho = fin[0];
             sc->rho = rho;
@@ -271,15 +290,20 @@ void cg_project_rho(hipStream_t s, const RedBuf &rb, T *z, const T *Q, int jq, c
                     long n, InnerScalars *sc, double *ycoef) {
     multidot<T>(s, rb, Q, n, jq, z, n, ycoef);
     const int g = grid_for(n);
-    unsigned *cnt = rb.counter + 5;
     double *p = rb.partials;
-    if (jq <= 1) k_cg_project_rho<T, 1><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
-    else if (jq <= 2) k_cg_project_rho<T, 2><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
-    else if (jq <= 4) k_cg_project_rho<T, 4><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
-    else if (jq <= 8) k_cg_project_rho<T, 8><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
-    else if (jq <= 16) k_cg_project_rho<T, 16><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
-    else if (jq <= 32) k_cg_project_rho<T, 32><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
-    else k_cg_project_rho<T, 64><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, cnt);
+#define JDHIP_PR(NC)                                                                    \
+    do {                                                                                \
+        k_cg_project_rho<T, NC><<<g, kBlock, 0, s>>>(z, Q, jq, r, n, sc, ycoef, p, 0);  \
+        k_cg_project_rho<T, NC><<<1, kBlock, 0, s>>>(z, Q, jq, r, 0, sc, ycoef, p, g);  \
+    } while (0)
+    if (jq <= 1) JDHIP_PR(1);
+    else if (jq <= 2) JDHIP_PR(2);
+    else if (jq <= 4) JDHIP_PR(4);
+    else if (jq <= 8) JDHIP_PR(8);
+    else if (jq <= 16) JDHIP_PR(16);
+    else if (jq <= 32) JDHIP_PR(32);
+    else JDHIP_PR(64);
+#undef JDHIP_PR
 }
 
 // direction (LSOLREVCOM label 3): z += (rho/rho1) p ; p = z
@@ -309,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_project_alpha(T *__restrict__ q,
                                                              const T *__restrict__ u,
                                                              const T *__restrict__ p, long n,
                                                              InnerScalars *sc, double *partials,
-                                                             unsigned *counter) {
+                                                             int nparts) {
     if (sc->flag) return;
     const T a = (T)(-sc->alprho);
     double acc[1] = {0.0};
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_project_alpha(T *__restrict__ q,
         acc[0] += (double)p[i] * (double)qi;
     }
     __shared__ double fin[1];
-    if (grid_reduce<1>(acc, 1, partials, counter, fin)) {
+    if (grid_reduce<1>(acc, 1, partials, nparts, fin)) {
         if (threadIdx.x == 0) {
             double pq = fin[0];
             sc->pq = pq;
@@ -331,8 +355,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_project_alpha(T *__restrict__ q,
 template <typename T>
 void cg_project_alpha(hipStream_t s, const RedBuf &rb, T *q, const T *u, const T *p, long n,
                       InnerScalars *sc) {
-    k_cg_project_alpha<T><<<grid_for(n), kBlock, 0, s>>>(q, u, p, n, sc, rb.partials,
-                                                         rb.counter + 6);
+    const int g = grid_for(n);
+    k_cg_project_alpha<T><<<g, kBlock, 0, s>>>(q, u, p, n, sc, rb.partials, 0);
+    k_cg_project_alpha<T><<<1, kBlock, 0, s>>>(q, u, p, 0, sc, rb.partials, g);
 }
 
 // PCG update: r -= alpha q ; x += alpha p ; ||r||^2, ||x||^2
@@ -342,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_posdef(T *__restrict__ r,
                                                              T *__restrict__ x,
                                                              const T *__restrict__ p, long n,
                                                              InnerScalars *sc, double *partials,
-                                                             unsigned *counter) {
+                                                             int nparts) {
     if (sc->flag) return;
     const T al = (T)sc->alpha;
     double acc[2] = {0.0, 0.0};
@@ -355,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_posdef(T *__restrict__ r,
         acc[1] += (double)xi * (double)xi;
     }
     __shared__ double fin[2];
-    if (grid_reduce<2>(acc, 2, partials, counter, fin)) {
+    if (grid_reduce<2>(acc, 2, partials, nparts, fin)) {
         if (threadIdx.x == 0) {
             sc->gk2 = fin[0];
             sc->xn2 = fin[1];
@@ -366,8 +391,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_posdef(T *__restrict__ r,
 template <typename T>
 void cg_update_posdef(hipStream_t s, const RedBuf &rb, T *r, const T *q, T *x, const T *p,
                       long n, InnerScalars *sc) {
-    k_cg_update_posdef<T><<<grid_for(n), kBlock, 0, s>>>(r, q, x, p, n, sc, rb.partials,
-                                                         rb.counter + 7);
+    const int g = grid_for(n);
+    k_cg_update_posdef<T><<<g, kBlock, 0, s>>>(r, q, x, p, n, sc, rb.partials, 0);
+    k_cg_update_posdef<T><<<1, kBlock, 0, s>>>(r, q, x, p, 0, sc, rb.partials, g);
 }
 
 // sQMR (LSOLREVCOM, JDaux.F:3167-3190): r -= alpha q, smoothing scalars
@@ -375,7 +401,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_qmr_update_r(T *__restrict__ r,
                                                          const T *__restrict__ q, long n,
                                                          InnerScalars *sc, double *partials,
-                                                         unsigned *counter) {
+                                                         int nparts) {
     if (sc->flag) return;
     const T al = (T)sc->alpha;
     double acc[1] = {0.0};
@@ -385,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_qmr_update_r(T *__restrict__ r,
         acc[0] += (double)ri * (double)ri;
     }
     __shared__ double fin[1];
-    if (grid_reduce<1>(acc, 1, partials, counter, fin)) {
+    if (grid_reduce<1>(acc, 1, partials, nparts, fin)) {
         if (threadIdx.x == 0) {
             double gk = sqrt(fin[0]);
             sc->gk2 = fin[0];
@@ -399,14 +425,16 @@ __global__ __launch_bounds__(kBlock) void k_qmr_update_r(T *__restrict__ r,
 }
 template <typename T>
 void qmr_update_r(hipStream_t s, const RedBuf &rb, T *r, const T *q, long n, InnerScalars *sc) {
-    k_qmr_update_r<T><<<grid_for(n), kBlock, 0, s>>>(r, q, n, sc, rb.partials, rb.counter + 7);
+    const int g = grid_for(n);
+    k_qmr_update_r<T><<<g, kBlock, 0, s>>>(r, q, n, sc, rb.partials, 0);
+    k_qmr_update_r<T><<<1, kBlock, 0, s>>>(r, q, 0, sc, rb.partials, g);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_qmr_update_x(T *__restrict__ x, T *__restrict__ x2,
                                                          const T *__restrict__ p, long n,
                                                          InnerScalars *sc, int iter,
-                                                         double *partials, unsigned *counter) {
+                                                         double *partials, int nparts) {
     if (sc->flag) return;
     const double al = sc->alpha;
     const bool first = iter <= 1;
@@ -428,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_qmr_update_x(T *__restrict__ x, T *_
         acc[0] += (double)xi * (double)xi;
     }
     __shared__ double fin[1];
-    if (grid_reduce<1>(acc, 1, partials, counter, fin))
+    if (grid_reduce<1>(acc, 1, partials, nparts, fin))
         if (threadIdx.x == 0) {
             sc->xn2 = fin[0];
             sc->rho1 = sc->rho;
@@ -437,8 +465,9 @@ __global__ __launch_bounds__(kBlock) void k_qmr_update_x(T *__restrict__ x, T *_
 template <typename T>
 void qmr_update_x(hipStream_t s, const RedBuf &rb, T *x, T *x2, const T *p, long n,
                   InnerScalars *sc, int iter) {
-    k_qmr_update_x<T><<<grid_for(n), kBlock, 0, s>>>(x, x2, p, n, sc, iter, rb.partials,
-                                                     rb.counter + 6);
+    const int g = grid_for(n);
+    k_qmr_update_x<T><<<g, kBlock, 0, s>>>(x, x2, p, n, sc, iter, rb.partials, 0);
+    k_qmr_update_x<T><<<1, kBlock, 0, s>>>(x, x2, p, 0, sc, iter, rb.partials, g);
 }
 
 #define JDHIP_INST(T)                                                                          \

@@ -17,41 +17,51 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Returns true in the last block, where fin[c] (shared) holds the totals.
+// Two-launch deterministic reduction.  Partial launch (nparts == 0): every
+// block writes its sums to partials[c * gridDim.x + blockIdx.x] and the call
+// returns false.  Finish launch (one block, nparts = number of partial blocks, the
+// kernel's element loop empty): fin[c] (shared) receives the totals in a
+// fixed order and the call returns true, so the caller's epilogue runs once.
+// (A single-launch "last block" ticket needs one device-scope atomic per
+// block on one address; across the 8 XCDs these serialise at roughly 0.1 us
+// each, which dominated every reduction kernel.)
 template <int NC>
-__device__ bool grid_reduce(double (&v)[NC], int nc, double *__restrict__ partials,
-                            unsigned *__restrict__ counter, double *fin) {
-    __shared__ double sh[NC][kBlock / 64];
-    __shared__ int amlast;
+__device__ bool grid_reduce(double (&v)[NC], int nc, double *__restrict__ partials, int nparts,
+                            double *fin) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (nparts == 0) {
+        __shared__ double sh[NC][kBlock / 64];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        if (c < nc) {
-            double t = wave_sum(v[c]);
-            if (lane == 0) sh[c][wid] = t;
+        for (int c = 0; c < NC; ++c) {
+            if (c < nc) {
+                double t = wave_sum(v[c]);
+                if (lane == 0) sh[c][wid] = t;
+            }
         }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nc) {
-        double s = 0.0;
+        __syncthreads();
+        if ((int)threadIdx.x < nc) {
+            double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) s += sh[threadIdx.x][w];
-        partials[(long)blockIdx.x * NC + threadIdx.x] = s;
+            for (int w = 0; w < kBlock / 64; ++w) s += sh[threadIdx.x][w];
+            partials[(long)threadIdx.x * gridDim.x + blockIdx.x] = s;
+        }
+        return false;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) amlast = (atomicAdd(counter, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!amlast) return false;
-    __threadfence();
-    for (int c = wid; c < nc; c += kBlock / 64) {
+    __shared__ double red[kBlock / 64];
+    for (int c = 0; c < nc; ++c) {
         double s = 0.0;
-        for (int b = lane; b < (int)gridDim.x; b += 64) s += partials[(long)b * NC + c];
+        for (int b = threadIdx.x; b < nparts; b += kBlock) s += partials[(long)c * nparts + b];
         s = wave_sum(s);
-        if (lane == 0) fin[c] = s;
+        if (lane == 0) red[wid] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+            fin[c] = t;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *counter = 0u;
     return true;
 }
 
