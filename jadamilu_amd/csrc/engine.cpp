@@ -8,7 +8,10 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "dense.hpp"
@@ -375,6 +378,26 @@ L40:
 }
 
 // ---------------------------------------------------------------------------
+// Progress line on stderr at most every 30 s of a solve when JDHIP_VERBOSE
+// is set (a solve of a large problem can run for minutes without returning).
+void JdEngine::progress(int matvecs, int converged) {
+    static const bool on = [] {
+        const char *v = std::getenv("JDHIP_VERBOSE");
+        return v && *v && *v != '0';
+    }();
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    if (!progress_started_) {
+        progress_started_ = true;
+        progress_t0_ = progress_last_ = now;
+        return;
+    }
+    if (std::chrono::duration<double>(now - progress_last_).count() < 30.0) return;
+    progress_last_ = now;
+    std::fprintf(stderr, "jdhip: %d MATVECs, %d eigenpairs converged, %.0f s\n", matvecs,
+                 converged, std::chrono::duration<double>(now - progress_t0_).count());
+}
+
 void JdEngine::step(int n, double *eigs, double *res, double *x, long lx, int *neig,
                     double sigma, int isearch, int ninit, int madspace, int *iter, double tol,
                     int *ijob, long *ndx1, long *ndx2, int iprint, int *info, double *gap) {
@@ -385,7 +408,10 @@ void JdEngine::step(int n, double *eigs, double *res, double *x, long lx, int *n
     const size_t nb = sizeof(double) * (size_t)n;
     auto D = [&](long off) { return X_ + off; };
 
-    if (*ijob == 0) goto L1;
+    if (*ijob == 0) {
+        progress_started_ = false;
+        goto L1;
+    }
     switch (rlbl) {
     case 1: goto L5;
     case 2: goto L225;
@@ -618,6 +644,7 @@ L20:  // expansion
     trs = std::sqrt(1.0 / 10);
     if (istag > 0) trs = 1.0 / 10;
     ++istep;
+    progress(*iter, I - 1);
     if (*iter >= mvmax - *neig - 1) goto L197;
     if (J >= modspace) {
         K = J;

@@ -11,6 +11,7 @@
 // inner iteration runs as a short chain of fused kernels with one host
 // synchronisation per step.
 #pragma once
+#include <chrono>
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -152,6 +153,11 @@ class JdEngine {
            st_tn = 0, st_un2 = 0, st_gkm1 = 0, st_alp = 0;
 
     unsigned long long rng_ = 0x2545F4914F6CDD1DULL;
+
+    // progress reporting (JDHIP_VERBOSE)
+    void progress(int matvecs, int converged);
+    bool progress_started_ = false;
+    std::chrono::steady_clock::time_point progress_t0_, progress_last_;
 };
 
 }  // namespace jdhip
