@@ -85,8 +85,8 @@ void schedule(int n, const std::vector<int> &ptr, const std::vector<int> &idx, b
 }
 
 // SELL-C-sigma copy of a CSR (C = 64 rows per slice, rows sorted by length
-// inside windows of kSigma rows) for the Jacobi sweeps.
-constexpr int kSigma = 1024;
+// inside windows of sigma rows; sigma <= 64 keeps the row order) for the
+// Jacobi sweeps.
 
 template <typename T>
 struct SellHost {
@@ -97,15 +97,17 @@ struct SellHost {
 
 template <typename T>
 SellHost<T> build_sell(int nrows, const std::vector<int> &ptr, const std::vector<int> &idx,
-                       const std::vector<T> &val) {
+                       const std::vector<T> &val, int sigma) {
     SellHost<T> h;
     std::vector<int> order(nrows);
-    for (int w0 = 0; w0 < nrows; w0 += kSigma) {
-        const int w1 = std::min(nrows, w0 + kSigma);
+    sigma = std::max(sigma, 64);
+    for (int w0 = 0; w0 < nrows; w0 += sigma) {
+        const int w1 = std::min(nrows, w0 + sigma);
         for (int r = w0; r < w1; ++r) order[r] = r;
-        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
-            return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b];
-        });
+        if (sigma > 64)
+            std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+                return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b];
+            });
         for (int s0 = w0; s0 < w1; s0 += 64) {
             int w = 0;
             for (int r = s0; r < std::min(w1, s0 + 64); ++r)
@@ -163,8 +165,10 @@ struct DevLevel {
 template <typename T>
 class IldlImpl : public IldlImplBase {
   public:
-    IldlImpl(DevCtx *ctx, IldlMultilevel<T> &&f, int sweeps, int storage_order, int xcd)
-        : ctx_(ctx), f_(std::move(f)), sweeps_(sweeps), storage_(storage_order), xcd_(xcd) {}
+    IldlImpl(DevCtx *ctx, IldlMultilevel<T> &&f, int sweeps, int storage_order, int xcd,
+             int sigma)
+        : ctx_(ctx), f_(std::move(f)), sweeps_(sweeps), storage_(storage_order), xcd_(xcd),
+          sigma_(sigma) {}
     ~IldlImpl() override {
         hipStreamSynchronize(ctx_->stream);
         for (void *p : allocs_) hipFree(p);
@@ -245,8 +249,8 @@ class IldlImpl : public IldlImplBase {
             D.lt_ptr = track(dev_upload(tp));
             D.lt_idx = track(dev_upload(ti));
             D.lt_val = track(dev_upload(tv));
-            D.sf = upload_sell(build_sell(nB, bp, bi, bv));
-            D.sb = upload_sell(build_sell(nB, tp, ti, tv));
+            D.sf = upload_sell(build_sell(nB, bp, bi, bv, sigma_));
+            D.sb = upload_sell(build_sell(nB, tp, ti, tv, sigma_));
             // dependency levels from the triangular (factor) numbering
             {
                 std::vector<int> rows, otp, oti;
@@ -259,9 +263,9 @@ class IldlImpl : public IldlImplBase {
                 for (int &r : rows) r = qinv[r];
                 D.bwd_rows = track(dev_upload(rows));
             }
-            D.sc = upload_sell(build_sell(nC, cp, ci, cv));
+            D.sc = upload_sell(build_sell(nC, cp, ci, cv, sigma_));
             transpose(nC, nB, cp, ci, cv, tp, ti, tv);
-            D.sct = upload_sell(build_sell(nB, tp, ti, tv));
+            D.sct = upload_sell(build_sell(nB, tp, ti, tv, sigma_));
             D.buf = track(dev_alloc<T>(n));
             D.out = track(dev_alloc<T>(n));
             D.t1 = track(dev_alloc<T>(std::max(nB, 1)));
@@ -541,6 +545,7 @@ class IldlImpl : public IldlImplBase {
     int sweeps_;
     int storage_;  // 0: factor order, 1: original-index order per level
     int xcd_;      // XCD-contiguous block order in the SELL kernels
+    int sigma_;    // SELL-C-sigma sorting window
     std::vector<DevLevel<T>> lv_;
     std::vector<void *> allocs_;
     T *dmat_ = nullptr, *dx_ = nullptr, *dy_ = nullptr;
@@ -596,12 +601,12 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     const int k0 = sweeps < 0 ? 0 : sweeps;
     if (h.single) {
         IldlMultilevel<float> f = h.ff;
-        auto *p = new IldlImpl<float>(ctx_, std::move(f), k0, storage_order, xcd_remap);
+        auto *p = new IldlImpl<float>(ctx_, std::move(f), k0, storage_order, xcd_remap, sell_sigma);
         impl_.reset(p);
         p->upload(result);
     } else {
         IldlMultilevel<double> f = h.fd;
-        auto *p = new IldlImpl<double>(ctx_, std::move(f), k0, storage_order, xcd_remap);
+        auto *p = new IldlImpl<double>(ctx_, std::move(f), k0, storage_order, xcd_remap, sell_sigma);
         impl_.reset(p);
         p->upload(result);
     }

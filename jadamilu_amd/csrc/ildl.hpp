@@ -84,8 +84,9 @@ class IldlPrecond {
     IldlResult result;
     // device layout: storage order of each level (0 factor order, 1 original
     // matrix index order) and XCD-contiguous block order in the sweeps
-    int storage_order = 0;
-    int xcd_remap = 0;
+    int storage_order = 1;
+    int xcd_remap = 1;
+    int sell_sigma = 256;  // SELL-C-sigma sorting window of the sweep copies
 
   private:
     DevCtx *ctx_;
