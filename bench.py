@@ -281,7 +281,11 @@ def main():
     ap.add_argument("--sweeps", type=int, default=-1,
                     help="ILDL triangular solves: 0 exact level-scheduled, k Jacobi sweeps, "
                          "-1 tuned at set-up")
-    ap.add_argument("--single", action="store_true", help="FP32 ILDL factor (ICNTL(5)=1)")
+    ap.add_argument("--single", action="store_true",
+                    help="mixed path: FP32 ILDL factor (ICNTL(5)=1) and FP32 inner solver")
+    ap.add_argument("--inner64", action="store_true",
+                    help="with --single: keep the inner solver in FP64 (FP32 factor only, the "
+                         "reference's ICNTL(5) semantics)")
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts of the batched workload")
     ap.add_argument("--kernels-only", action="store_true",
@@ -333,6 +337,7 @@ def main():
     single = 1 if args.single else 0
     icntl = (0, -1, 0, 0, single)  # timed solves: keep the resident factor
     ctx.set_option("ildl_sweeps", args.sweeps)
+    ctx.set_option("inner_fp32", 0 if args.inner64 else 1)
 
     def one_solve(icntl2):
         X[:n].copy_(x0_dev)
@@ -477,7 +482,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner/factor + f64 outer" if args.single else "f64",
+        "dtype": ("f32 factor + f64 inner/outer" if args.inner64 else
+                  "f32 inner/factor + f64 outer") if args.single else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz_upper": nnz_u, "neig": neig,
                    "tol": args.tol, "madspace": madspace, "isearch": isearch, "sigma": sigma,
