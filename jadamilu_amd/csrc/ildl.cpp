@@ -555,6 +555,31 @@ class IldlImpl : public IldlImplBase {
 
 }  // namespace
 
+// Copy of a double factor with every stored value rounded to float.
+static IldlMultilevel<float> round_to_float(const IldlMultilevel<double> &d) {
+    IldlMultilevel<float> f;
+    for (const auto &L : d.levels) {
+        IldlLevel<float> F;
+        F.n = L.n;
+        F.nB = L.nB;
+        F.perm = L.perm;
+        F.scal.assign(L.scal.begin(), L.scal.end());
+        F.lb_ptr = L.lb_ptr;
+        F.lb_idx = L.lb_idx;
+        F.lb_val.assign(L.lb_val.begin(), L.lb_val.end());
+        F.lc_ptr = L.lc_ptr;
+        F.lc_idx = L.lc_idx;
+        F.lc_val.assign(L.lc_val.begin(), L.lc_val.end());
+        F.d.assign(L.d.begin(), L.d.end());
+        f.levels.push_back(std::move(F));
+    }
+    f.m = d.m;
+    f.dinv = d.dinv;
+    f.nnz_factor = d.nnz_factor;
+    f.nneg = d.nneg;
+    return f;
+}
+
 void ildl_factor_host(int n, const int *ia, const int *ja, const double *a, double shift,
                       const IldlOptions &opt, IldlHost &out) {
     auto t0 = std::chrono::steady_clock::now();
@@ -572,8 +597,15 @@ void ildl_factor_host(int n, const int *ia, const int *ja, const double *a, doub
     prm.dense_max = opt.dense_max;
     IldlStats st;
     out.single = opt.single;
-    if (opt.single) st = ildl_factor<float>(A, prm, out.ff, nnzu);
-    else st = ildl_factor<double>(A, prm, out.fd, nnzu);
+    // The factor is always computed in double; the mixed path stores and
+    // applies it in float.  (A float factorisation of the strongly indefinite
+    // Anderson matrices loses the Schur complements: at L=100 its
+    // preconditioner no longer lets the iteration converge.)
+    st = ildl_factor<double>(A, prm, out.fd, nnzu);
+    if (opt.single && st.ierr == 0) {
+        out.ff = round_to_float(out.fd);
+        out.fd = IldlMultilevel<double>();
+    }
     res.ierr = st.ierr;
     res.nlev = st.nlev;
     res.nnz = st.nnz;
