@@ -189,6 +189,10 @@ int jdhip_ildl_stats(jdhip_ildl *f, double *fill, int *nneg, double *condest, do
 int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps);
 /* dependency levels of the uploaded factor and algorithmic bytes per apply */
 int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels, long *apply_bytes);
+/* Triangular-solve mode of each sparse level of ctx's ILDL preconditioner
+ * (0 exact level-scheduled, k Jacobi sweeps) into out[0..cap-1]; returns the
+ * number of levels, or -1 without an ILDL preconditioner. */
+int jdhip_ildl_level_sweeps(jdhip_ctx *ctx, int *out, int cap);
 /* Diagnostic: time reps launches of one Jacobi sweep over the first level's
  * L_B (the largest single kernel of the preconditioner apply) on ctx's stream
  * with HIP events; *alg_bytes = its algorithmic bytes per launch.  No

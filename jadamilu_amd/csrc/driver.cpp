@@ -974,6 +974,14 @@ int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels,
     return 0;
 }
 
+int jdhip_ildl_level_sweeps(jdhip_ctx *ctx, int *out, int cap) {
+    Solver &S = *ctx->S;
+    if (!S.ildl || !S.ildl->ready()) return -1;
+    std::vector<int> v = S.ildl->impl()->level_sweeps();
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size();
+}
+
 int jdhip_ildl_time_sweep(jdhip_ctx *ctx, int reps, double *avg_ms, long *alg_bytes) {
     try {
         Solver &S = *ctx->S;

@@ -141,6 +141,17 @@ struct SellDev {
     T *val = nullptr;
 };
 
+// Lanes per row for a product with nnz entries over rows rows: one lane per
+// row (SELL) for short rows, CSR lane groups for long rows or small levels.
+inline int pick_lanes(long nnz, int rows) {
+    if (rows <= 0) return 0;
+    const double avg = (double)nnz / rows;
+    if (avg >= 96.0) return 64;
+    if (avg >= 24.0) return 16;
+    if (rows < 65536 && avg >= 6.0) return 4;
+    return 0;
+}
+
 template <typename T>
 struct DevLevel {
     int n = 0, nB = 0, nC = 0;
@@ -159,6 +170,10 @@ struct DevLevel {
     T *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;
     long nnz_lb = 0, nnz_lc = 0;
     SellDev<T> sf, sb;  // sweep copies of L_B (forward) and L_B^T (backward)
+    int k = 0;          // this level's triangular solves: 0 exact, k Jacobi sweeps
+    // kernel form of each product: 0 SELL (one lane per row), else CSR with
+    // that many lanes per row (long rows)
+    int wf = 0, wb = 0, wc = 0, wct = 0;
     SellDev<T> sc, sct; // L_CB and L_CB^T
 };
 
@@ -264,8 +279,17 @@ class IldlImpl : public IldlImplBase {
                 D.bwd_rows = track(dev_upload(rows));
             }
             D.sc = upload_sell(build_sell(nC, cp, ci, cv, sigma_));
+            D.lc_ptr = track(dev_upload(cp));
+            D.lc_idx = track(dev_upload(ci));
+            D.lc_val = track(dev_upload(cv));
+            D.wc = pick_lanes((long)ci.size(), nC);
+            D.wf = D.wb = pick_lanes((long)bi.size(), nB);
             transpose(nC, nB, cp, ci, cv, tp, ti, tv);
             D.sct = upload_sell(build_sell(nB, tp, ti, tv, sigma_));
+            D.lct_ptr = track(dev_upload(tp));
+            D.lct_idx = track(dev_upload(ti));
+            D.lct_val = track(dev_upload(tv));
+            D.wct = pick_lanes((long)ti.size(), nB);
             D.buf = track(dev_alloc<T>(n));
             D.out = track(dev_alloc<T>(n));
             D.t1 = track(dev_alloc<T>(std::max(nB, 1)));
@@ -298,6 +322,7 @@ class IldlImpl : public IldlImplBase {
             dx_ = track(dev_alloc<T>(m));
             dy_ = track(dev_alloc<T>(m));
         }
+        for (auto &D : lv_) D.k = sweeps_;
         JDHIP_CHECK(hipDeviceSynchronize());
     }
 
@@ -306,7 +331,8 @@ class IldlImpl : public IldlImplBase {
         const int *ptr = fwd ? D.lb_ptr : D.lt_ptr;
         const int *idx = fwd ? D.lb_idx : D.lt_idx;
         const T *val = fwd ? D.lb_val : D.lt_val;
-        if (sweeps_ <= 0) {
+        const int nsw = D.k;
+        if (nsw <= 0) {
             const std::vector<int> &lp = fwd ? D.fwd_ptr : D.bwd_ptr;
             const int *rows = fwd ? D.fwd_rows : D.bwd_rows;
             for (size_t l = 1; l + 1 < lp.size(); ++l)  // level 0 rows have no entries
@@ -318,20 +344,21 @@ class IldlImpl : public IldlImplBase {
         // every row reads only its own b entry and all x entries come from
         // the other buffer (needs at least 2 sweeps; 1 sweep goes via t1).
         const SellDev<T> &M = fwd ? D.sf : D.sb;
+        const int lanes = fwd ? D.wf : D.wb;
         const T *xin = D.buf;
         T *bufs[2] = {D.t1, D.t2};
-        for (int k = 0; k < sweeps_; ++k) {
-            const bool last = (k == sweeps_ - 1) && sweeps_ >= 2;
+        for (int k = 0; k < nsw; ++k) {
+            const bool last = (k == nsw - 1) && nsw >= 2;
             T *xout = last ? D.buf : bufs[k & 1];
-            ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, D.buf,
-                                      xin, xout, xcd_);
+            if (lanes > 0)
+                ildl_jacobi_csr<T>(s, lanes, D.nB, ptr, idx, val, D.buf, xin, xout);
+            else
+                ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col,
+                                          D.buf, xin, xout, xcd_);
             xin = xout;
         }
-        if (sweeps_ == 1)
+        if (nsw == 1)
             JDHIP_CHECK(hipMemcpyAsync(D.buf, D.t1, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
-        (void)ptr;
-        (void)idx;
-        (void)val;
     }
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
@@ -347,8 +374,13 @@ class IldlImpl : public IldlImplBase {
 
     int sweeps() const override { return sweeps_; }
 
+    void set_all(int k) {
+        sweeps_ = k;
+        for (auto &D : lv_) D.k = k;
+    }
+
     int tune_sweeps(double tol) override {
-        sweeps_ = 0;
+        set_all(0);
         if (lv_.empty()) return 0;
         const int n = lv_[0].n;
         hipStream_t s = ctx_->stream;
@@ -376,31 +408,60 @@ class IldlImpl : public IldlImplBase {
             JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             return ms / reps;
         };
-        float best_t = timed(2);
+        const float t_exact = timed(2);
         JDHIP_CHECK(hipMemcpy(h0.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost));
         double n0 = 0.0;
         for (double v : h0) n0 += v * v;
         n0 = std::sqrt(n0);
-        int best = 0;
-        for (int k : {2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64}) {
-            sweeps_ = k;
-            float t = timed(2);
-            if (t >= best_t) break;  // more sweeps only cost more
+        auto close = [&]() {  // apply of the current setting within tol of exact?
+            apply(z, r);
             JDHIP_CHECK(hipMemcpy(hk.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost));
             double e = 0.0;
             for (int i = 0; i < n; ++i) e += (hk[i] - h0[i]) * (hk[i] - h0[i]);
-            if (std::sqrt(e) <= tol * n0) {
-                best = k;
-                best_t = t;
+            return std::sqrt(e) <= tol * n0;
+        };
+        // 1) one sweep count for all levels: the smallest that is accurate
+        int kg = 0;
+        for (int k : {2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64}) {
+            set_all(k);
+            if (close()) {
+                kg = k;
                 break;
             }
         }
-        sweeps_ = best;
+        if (kg > 0) {
+            // 2) per level (largest first), halve the count while the whole
+            //    apply stays within tol
+            std::vector<int> order(lv_.size());
+            for (size_t l = 0; l < lv_.size(); ++l) order[l] = (int)l;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int a, int b) { return lv_[a].nB > lv_[b].nB; });
+            for (int l : order) {
+                while (lv_[l].k > 1) {
+                    const int keep = lv_[l].k;
+                    lv_[l].k = keep / 2;
+                    if (!close()) {
+                        lv_[l].k = keep;
+                        break;
+                    }
+                }
+            }
+            sweeps_ = 0;
+            for (auto &D : lv_) sweeps_ = std::max(sweeps_, D.k);
+            // 3) keep the sweeps only if they beat the exact solve
+            if (timed(2) >= t_exact) set_all(0);
+        }
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         hipFree(r);
         hipFree(z);
-        return best;
+        return sweeps_;
+    }
+
+    std::vector<int> level_sweeps() const override {
+        std::vector<int> v;
+        for (const auto &D : lv_) v.push_back(D.k);
+        return v;
     }
 
     void apply(double *z, const double *r) override { apply_io<double>(z, r); }
@@ -425,7 +486,9 @@ class IldlImpl : public IldlImplBase {
                 ildl_gather<T, T>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
             }
             tri_solve(D, true);
-            if (D.nC > 0)
+            if (D.nC > 0 && D.wc > 0)
+                ildl_sub_csr<T>(s, D.wc, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, D.buf, D.buf + D.nB);
+            else if (D.nC > 0)
                 ildl_sub_sell<T>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
                                  D.sc.col, D.buf, D.buf + D.nB, xcd_);
             ildl_dscale<T>(s, D.nB, D.dinv, D.buf);
@@ -438,7 +501,9 @@ class IldlImpl : public IldlImplBase {
         }
         for (int l = L - 1; l >= 0; --l) {
             DevLevel<T> &D = lv_[l];
-            if (D.nC > 0)
+            if (D.nC > 0 && D.wct > 0)
+                ildl_sub_csr<T>(s, D.wct, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, zc, D.buf);
+            else if (D.nC > 0)
                 ildl_sub_sell<T>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
                                  D.sct.val, D.sct.col, zc, D.buf, xcd_);
             tri_solve(D, false);
@@ -478,9 +543,9 @@ class IldlImpl : public IldlImplBase {
         // indices, row map, b, x, result), L_CB and L_CB^T once,
         // gather/scatter (perm + scal + in/out), D^-1, dense inverse.
         const long s = sizeof(T);
-        const long k = sweeps_ > 0 ? sweeps_ : 1;
         long b = 0;
         for (const auto &D : lv_) {
+            const long k = D.k > 0 ? D.k : 1;
             b += 2 * k * ((s + 4) * D.nnz_lb + 4L * D.nB + 3 * s * D.nB);
             b += 2 * ((s + 4) * D.nnz_lc + 4L * (D.nC + D.nB + 2) + s * (D.nB + D.nC));
             b += 2 * (4L + s) * D.n + 2 * s * D.n + 2 * s * D.nB;

@@ -55,6 +55,7 @@ class IldlImplBase {
     // largest single kernel of the apply) on the context stream.
     virtual int time_sweep(int reps, double *avg_ms, long *alg_bytes) = 0;
     virtual int sweeps() const = 0;
+    virtual std::vector<int> level_sweeps() const = 0;
 };
 
 // Host factor (no device needed).

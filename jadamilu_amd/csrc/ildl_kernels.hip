@@ -141,6 +141,39 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
     }
 }
 
+// CSR forms for matrices with long rows (LPR lanes per row, xor-butterfly
+// reduction inside the lane group): with one lane per row such levels keep
+// too few waves in flight and each lane walks its row serially.
+template <typename T, int LPR>
+__global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restrict__ ptr,
+                                                   const int *__restrict__ idx,
+                                                   const T *__restrict__ val, const T *b,
+                                                   const T *xin, T *xout) {
+    const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
+        T s = T(0);
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += val[k] * xin[idx[k]];
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
+        if (l == 0) xout[r] = b[r] - s;
+    }
+}
+
+template <typename T, int LPR>
+__global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict__ ptr,
+                                                const int *__restrict__ idx,
+                                                const T *__restrict__ val,
+                                                const T *__restrict__ x, T *__restrict__ y) {
+    const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
+        T s = T(0);
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += val[k] * x[idx[k]];
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
+        if (l == 0) y[r] -= s;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kB) void k_spmv_sub(int nrows, const int *__restrict__ ptr,
                                                  const int *__restrict__ idx,
@@ -218,6 +251,34 @@ void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                                                                       val, col, x, y, xcd);
 }
 template <typename T>
+void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
+                     const T *val, const T *b, const T *xin, T *xout) {
+    if (nrows <= 0) return;
+    if (lpr >= 64)
+        k_jacobi_csr<T, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
+                                                                              val, b, xin, xout);
+    else if (lpr >= 16)
+        k_jacobi_csr<T, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
+                                                                              val, b, xin, xout);
+    else
+        k_jacobi_csr<T, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
+                                                                            b, xin, xout);
+}
+template <typename T>
+void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
+                  const T *val, const T *x, T *y) {
+    if (nrows <= 0) return;
+    if (lpr >= 64)
+        k_sub_csr<T, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
+                                                                           x, y);
+    else if (lpr >= 16)
+        k_sub_csr<T, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
+                                                                           x, y);
+    else
+        k_sub_csr<T, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val, x,
+                                                                         y);
+}
+template <typename T>
 void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
                    const T *x, T *y) {
     if (nrows <= 0) return;
@@ -244,6 +305,10 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
                                             const T *, T *, int);                            \
     template void ildl_sub_sell<T>(hipStream_t, int, const long *, const int *, const int *,  \
                                    const T *, const int *, const T *, T *, int);             \
+    template void ildl_jacobi_csr<T>(hipStream_t, int, int, const int *, const int *,         \
+                                     const T *, const T *, const T *, T *);                  \
+    template void ildl_sub_csr<T>(hipStream_t, int, int, const int *, const int *, const T *, \
+                                  const T *, T *);                                           \
     template void ildl_spmv_sub<T>(hipStream_t, int, const int *, const int *, const T *,     \
                                    const T *, T *);                                          \
     template void ildl_dscale<T>(hipStream_t, int, const T *, T *);                          \

@@ -34,6 +34,13 @@ template <typename T>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                             const int *rowid, const T *val, const int *col, const T *b,
                             const T *xin, T *xout, int xcd);
+// CSR forms with lpr (4, 16 or 64) lanes per row, for matrices with long rows
+template <typename T>
+void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
+                     const T *val, const T *b, const T *xin, T *xout);
+template <typename T>
+void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
+                  const T *val, const T *x, T *y);
 // y[r] -= sum_j M(r, j) x[j] over the rows of a SELL-C-sigma matrix
 template <typename T>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,

@@ -268,6 +268,12 @@ class Context:
             raise RuntimeError(self.error() or "no ILDL preconditioner")
         return ms.value, nb.value
 
+    def ildl_level_sweeps(self):
+        """Per sparse level: 0 exact triangular solves, k Jacobi sweeps."""
+        buf = (C.c_int * 64)()
+        k = self._L.jdhip_ildl_level_sweeps(self.h, buf, 64)
+        return list(buf[: max(k, 0)])
+
     def ildl_device_info(self):
         f, b, nb = C.c_int(), C.c_int(), C.c_long()
         if self._L.jdhip_ildl_device_info(self.h, C.byref(f), C.byref(b), C.byref(nb)) != 0:

@@ -65,11 +65,16 @@ def test_device_apply_vs_oracle(gen, shift, spd, single):
     ctx.close()
 
 
-def test_sweep_mode_converges_to_exact():
+@pytest.mark.parametrize("shape", [(12, 11, 10), (48, 40, 36)])
+def test_sweep_mode_converges_to_exact(shape):
+    """Jacobi sweeps reach the exact apply after as many sweeps as dependency
+    levels (the strictly triangular part is nilpotent).  The small shape runs
+    the CSR lane-group kernels, the larger one the SELL-C-sigma kernels on
+    its first level."""
     from jadamilu_amd.ildl import IldlFactor
     from jadamilu_amd.jd import Context
     _dev()
-    ia, ja, a = M.laplace((12, 11, 10))
+    ia, ja, a = M.laplace(shape)
     n = ia.size - 1
     f = IldlFactor(n, ia, ja, a, droptol=1e-3, spd=True, dense_max=100)
     ctx = Context(0)
