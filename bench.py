@@ -47,7 +47,9 @@ WORKLOADS = {
     "anderson_40": ("anderson", (40, 16.5, 0), 4, 25, 2, 0.0, 25.0),
     # 256 independent Anderson problems, L uniform in {16..48}, W uniform in
     # [12, 20] (numpy PCG64 seed 0), 8 eigenpairs nearest 0 each, FP32 factor
-    # and inner solver (ICNTL(5)=1), solved by jdhip_pjd_batch
+    # and inner solver (ICNTL(5)=1), solved by jdhip_pjd_batch; ICNTL(2)=0 so
+    # that a factor exceeding MEM is retried with a larger DROPTOL (the
+    # weakly disordered instances, W near 12, fill up to ~50 at 1e-3)
     "anderson_batch256": ("batch", (256, 16, 48, 12.0, 20.0, 0), 8, 25, 2, 0.0, 25.0),
     # gradients of sum_k w_k lambda_k (w ~ N(0,1), seed 40) over the 4
     # eigenvalues nearest 0 of Anderson L=40, W=16.5: dV (n) and dA (CSR),
@@ -140,7 +142,7 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
         ntot += ia.size - 1
         probs.append(dict(ia=ia, ja=ja, a=a, neig=neig, isearch=isearch, sigma=sigma,
                           madspace=madspace, maxiter=args.maxiter, tol=args.tol,
-                          droptol=args.droptol, mem=mem, icntl=(0, 1, 0, 0, 1)))
+                          droptol=args.droptol, mem=mem, icntl=(0, 0, 0, 0, 1)))
     print(f"# batch of {count} problems, total n {ntot}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         jd.pjd_batch(probs, workers=args.workers)

@@ -61,6 +61,7 @@ class IldlImplBase {
 // Host factor (no device needed).
 struct IldlHost {
     bool single = false;
+    bool spd = false;  // factor converted to |D| (ISEARCH <= 1)
     IldlMultilevel<double> fd;
     IldlMultilevel<float> ff;
     IldlResult res;
