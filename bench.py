@@ -436,21 +436,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        sample = args.cpu_sample_matvecs
-        if sample <= 0:
-            # ~15 s of single-core work: estimate cost of one MATVEC step
-            est_step_s = 4e-9 * (nnz_u * 2 + 30 * n)
-            if args.precond == "ildl":
-                est_step_s += 8e-9 * 2 * (r.stats.get("fill") or 2.0) * nnz_u
-            sample = int(max(10, min(r.iter, 15.0 / max(est_step_s, 1e-9))))
         fac_s = 0.0
         if args.precond == "diag":
             mv = O.matvec_upper(ia, ja, a)
-            tc = time.perf_counter()
-            ro = O.pjdrevcom_diag(n, diag, mv, neig=neig, isearch=isearch, sigma=sigma,
-                                  ninit=1, x0=x0, madspace=madspace, maxiter=sample,
-                                  tol=args.tol, icntl=(0, 1, 0, 0, 0))
-            cpu_s = time.perf_counter() - tc
+
+            def cpu_run(maxit):
+                return O.pjdrevcom_diag(n, diag, mv, neig=neig, isearch=isearch, sigma=sigma,
+                                        ninit=1, x0=x0, madspace=madspace, maxiter=maxit,
+                                        tol=args.tol, icntl=(0, 1, 0, 0, 0))
         else:
             from jadamilu_amd.ildl import IldlFactor
             shift0 = (r_setup.stats["setup_shifts"][0] if r_setup.stats.get("setup_shifts")
@@ -460,10 +453,21 @@ def main():
                              spd=(isearch <= 1), single=args.single)
             ex = fac.export()
             fac_s = time.perf_counter() - tf
+
+            def cpu_run(maxit):
+                return O.pjd_ildl(n, ia, ja, a, ex, neig=neig, isearch=isearch, sigma=sigma,
+                                  ninit=1, x0=x0, madspace=madspace, maxiter=maxit,
+                                  tol=args.tol)
+        sample = args.cpu_sample_matvecs
+        if sample <= 0:
+            # size the sample to ~15 s of single-core work from a short probe
             tc = time.perf_counter()
-            ro = O.pjd_ildl(n, ia, ja, a, ex, neig=neig, isearch=isearch, sigma=sigma, ninit=1,
-                            x0=x0, madspace=madspace, maxiter=sample, tol=args.tol)
-            cpu_s = time.perf_counter() - tc
+            rp = cpu_run(8)
+            per = (time.perf_counter() - tc) / max(rp.iter, 1)
+            sample = int(max(10, min(r.iter, 15.0 / max(per, 1e-9))))
+        tc = time.perf_counter()
+        ro = cpu_run(sample)
+        cpu_s = time.perf_counter() - tc
         per_mv = cpu_s / max(ro.iter, 1)
         est_solve = per_mv * r.iter
         cpu = {"value": r.neig / est_solve, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
