@@ -62,6 +62,7 @@ struct Solver {
     // 128^3 Laplacian: 1.47 ms against 1.84 ms for factor order / window 1024)
     int ildl_storage = 1, ildl_xcd = 1;
     int ildl_sigma = 256;
+    bool ildl_graph = true;  // replay the preconditioner apply from HIP graphs
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -502,6 +503,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         ildl->storage_order = ildl_storage;
         ildl->xcd_remap = ildl_xcd;
         ildl->sell_sigma = ildl_sigma;
+        ildl->graphs = ildl_graph;
         ildl->attach(h, ildl_sweeps);
         ildl_info = ildl->result;
         totaltime =
@@ -944,6 +946,10 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_storage") S.ildl_storage = (int)value;
     else if (k == "ildl_xcd") S.ildl_xcd = (int)value;
     else if (k == "ildl_sigma") S.ildl_sigma = (int)value;
+    else if (k == "ildl_graph") {
+        S.ildl_graph = value != 0.0;
+        if (S.ildl && S.ildl->ready()) S.ildl->impl()->set_graphs(S.ildl_graph);
+    }
     else return -2;
     return 0;
 }
@@ -1100,6 +1106,7 @@ int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
         S.ildl->storage_order = S.ildl_storage;
         S.ildl->xcd_remap = S.ildl_xcd;
         S.ildl->sell_sigma = S.ildl_sigma;
+        S.ildl->graphs = S.ildl_graph;
         S.ildl->attach(f->h, sweeps);
         S.factdgl = -1;
         S.n = f->h.single ? (f->h.ff.levels.empty() ? f->h.ff.m : f->h.ff.levels[0].n)

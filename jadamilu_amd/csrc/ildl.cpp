@@ -186,6 +186,7 @@ class IldlImpl : public IldlImplBase {
           sigma_(sigma) {}
     ~IldlImpl() override {
         hipStreamSynchronize(ctx_->stream);
+        clear_graphs();
         for (void *p : allocs_) hipFree(p);
     }
 
@@ -377,11 +378,13 @@ class IldlImpl : public IldlImplBase {
     void set_all(int k) {
         sweeps_ = k;
         for (auto &D : lv_) D.k = k;
+        clear_graphs();
     }
 
     int tune_sweeps(double tol) override {
         set_all(0);
         if (lv_.empty()) return 0;
+        tuning_ = true;  // per-level counts change between applies
         const int n = lv_[0].n;
         hipStream_t s = ctx_->stream;
         std::vector<double> hr(n), h0(n), hk(n);
@@ -451,6 +454,8 @@ class IldlImpl : public IldlImplBase {
             // 3) keep the sweeps only if they beat the exact solve
             if (timed(2) >= t_exact) set_all(0);
         }
+        tuning_ = false;
+        clear_graphs();
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         hipFree(r);
@@ -464,8 +469,48 @@ class IldlImpl : public IldlImplBase {
         return v;
     }
 
-    void apply(double *z, const double *r) override { apply_io<double>(z, r); }
-    void apply_f(float *z, const float *r) override { apply_io<float>(z, r); }
+    void apply(double *z, const double *r) override { apply_cached<double>(z, r); }
+    void apply_f(float *z, const float *r) override { apply_cached<float>(z, r); }
+
+    // The apply is a fixed sequence of 30-100 launches (more in the exact
+    // mode); it is captured once per (output, input) pair into a HIP graph
+    // and replayed with a single launch.  The sequence records kernels and
+    // device-to-device copies only.  Any change of the sweep setting drops
+    // the cached graphs; tuning runs uncached.
+    template <typename TV>
+    void apply_cached(TV *z, const TV *r) {
+        if (!graphs_on_ || tuning_) {
+            apply_io<TV>(z, r);
+            return;
+        }
+        const int type = sizeof(TV) == 8 ? 0 : 1;
+        for (auto &g : graphs_)
+            if (g.z == (const void *)z && g.r == (const void *)r && g.type == type) {
+                JDHIP_CHECK(hipGraphLaunch(g.exec, ctx_->stream));
+                return;
+            }
+        if (graphs_.size() >= 16) {  // unexpected variety of buffers: no caching
+            apply_io<TV>(z, r);
+            return;
+        }
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        JDHIP_CHECK(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
+        apply_io<TV>(z, r);
+        JDHIP_CHECK(hipStreamEndCapture(ctx_->stream, &graph));
+        JDHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        JDHIP_CHECK(hipGraphDestroy(graph));
+        graphs_.push_back({(const void *)z, (const void *)r, type, exec});
+        JDHIP_CHECK(hipGraphLaunch(exec, ctx_->stream));
+    }
+    void clear_graphs() {
+        for (auto &g : graphs_) hipGraphExecDestroy(g.exec);
+        graphs_.clear();
+    }
+    void set_graphs(bool on) override {
+        graphs_on_ = on;
+        clear_graphs();
+    }
 
     template <typename TV>
     void apply_io(TV *z, const TV *r) {
@@ -607,6 +652,13 @@ class IldlImpl : public IldlImplBase {
 
     DevCtx *ctx_;
     IldlMultilevel<T> f_;
+    struct CachedGraph {
+        const void *z, *r;
+        int type;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> graphs_;
+    bool graphs_on_ = true, tuning_ = false;
     int sweeps_;
     int storage_;  // 0: factor order, 1: original-index order per level
     int xcd_;      // XCD-contiguous block order in the SELL kernels
@@ -717,6 +769,7 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     // random vector stalled the iteration), so there the bound is 1e-4.
     if (sweeps < 0) impl_->tune_sweeps(h.spd ? 0.03 : 1e-4);
     result.sweeps = impl_->sweeps();
+    impl_->set_graphs(graphs);
 }
 
 IldlResult IldlPrecond::setup(int n, const int *ia, const int *ja, const double *a,

@@ -56,6 +56,8 @@ class IldlImplBase {
     virtual int time_sweep(int reps, double *avg_ms, long *alg_bytes) = 0;
     virtual int sweeps() const = 0;
     virtual std::vector<int> level_sweeps() const = 0;
+    // replay the apply from captured HIP graphs (default on)
+    virtual void set_graphs(bool on) = 0;
 };
 
 // Host factor (no device needed).
@@ -89,6 +91,7 @@ class IldlPrecond {
     int storage_order = 1;
     int xcd_remap = 1;
     int sell_sigma = 256;  // SELL-C-sigma sorting window of the sweep copies
+    bool graphs = true;    // replay the apply from captured HIP graphs
 
   private:
     DevCtx *ctx_;
