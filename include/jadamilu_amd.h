@@ -154,7 +154,9 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
  * sweeps only where they reproduce the exact apply to 3 %),
  * "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum
  * degree, the default), "ildl_dense_max"
- * (largest final level factored densely), "spmv_lanes".  0 on success. */
+ * (largest final level factored densely), "spmv_lanes", "ildl_graph" (1:
+ * replay the preconditioner apply from captured HIP graphs; default 0),
+ * "ildl_storage", "ildl_xcd", "ildl_sigma" (device layout of the factor).  0 on success. */
 int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value);
 
 /* z = P^-1 r with the preconditioner currently held by ctx (diagonal or

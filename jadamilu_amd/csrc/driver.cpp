@@ -62,7 +62,12 @@ struct Solver {
     // 128^3 Laplacian: 1.47 ms against 1.84 ms for factor order / window 1024)
     int ildl_storage = 1, ildl_xcd = 1;
     int ildl_sigma = 256;
-    bool ildl_graph = true;  // replay the preconditioner apply from HIP graphs
+    // replay the preconditioner apply from captured HIP graphs: off by default
+    // (no gain measured on the 128^3 Laplacian, whose apply is bandwidth- and
+    // not launch-bound; and a capture on one worker's stream is invalidated by
+    // synchronous HIP calls of other threads in the batched driver, whose
+    // blocking streams synchronise with the null stream)
+    bool ildl_graph = false;
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;

@@ -658,7 +658,7 @@ class IldlImpl : public IldlImplBase {
         hipGraphExec_t exec;
     };
     std::vector<CachedGraph> graphs_;
-    bool graphs_on_ = true, tuning_ = false;
+    bool graphs_on_ = false, tuning_ = false;
     int sweeps_;
     int storage_;  // 0: factor order, 1: original-index order per level
     int xcd_;      // XCD-contiguous block order in the SELL kernels

@@ -91,7 +91,7 @@ class IldlPrecond {
     int storage_order = 1;
     int xcd_remap = 1;
     int sell_sigma = 256;  // SELL-C-sigma sorting window of the sweep copies
-    bool graphs = true;    // replay the apply from captured HIP graphs
+    bool graphs = false;   // replay the apply from captured HIP graphs
 
   private:
     DevCtx *ctx_;
