@@ -46,6 +46,8 @@ extern "C" int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, 
     auto worker = [&]() {
         jdhip_ctx *ctx = jdhip_create(device);
         if (!ctx) return;  // this worker cannot run; the others drain the queue
+        // the workers already run factorisations in parallel
+        jdhip_set_option(ctx, "ildl_threads", 1);
         double *x = nullptr;
         long xcap = 0;
         std::vector<int> ia, ja;

@@ -68,6 +68,7 @@ struct Solver {
     // synchronous HIP calls of other threads in the batched driver, whose
     // blocking streams synchronise with the null stream)
     bool ildl_graph = false;
+    int ildl_threads = 0;  // host threads of the factorisation (0: all cores, <= 16)
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -463,6 +464,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         opt.sweeps = ildl_sweeps;
         opt.ordering = ildl_ordering;
         opt.dense_max = ildl_dense_max;
+        opt.threads = ildl_threads;
         IldlHost h;
         ildl_factor_host(n, ia, ja, a, shift0, opt, h);
         const IldlResult &r = h.res;
@@ -951,6 +953,7 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_storage") S.ildl_storage = (int)value;
     else if (k == "ildl_xcd") S.ildl_xcd = (int)value;
     else if (k == "ildl_sigma") S.ildl_sigma = (int)value;
+    else if (k == "ildl_threads") S.ildl_threads = (int)value;
     else if (k == "ildl_graph") {
         S.ildl_graph = value != 0.0;
         if (S.ildl && S.ildl->ready()) S.ildl->impl()->set_graphs(S.ildl_graph);

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 #include "ildl_kernels.hpp"
 
@@ -712,6 +713,9 @@ void ildl_factor_host(int n, const int *ia, const int *ja, const double *a, doub
     prm.spd = opt.spd;
     prm.ordering = opt.ordering;
     prm.dense_max = opt.dense_max;
+    prm.threads = opt.threads > 0
+                      ? opt.threads
+                      : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     IldlStats st;
     out.single = opt.single;
     out.spd = opt.spd;

@@ -18,6 +18,7 @@ struct IldlOptions {
     int sweeps = -1;     // 0: exact level-scheduled; k > 0: k Jacobi sweeps; -1: tuned
     int ordering = 2;  // 0 natural, 1 RCM, 2 AMD
     int dense_max = 1500;
+    int threads = 0;     // host threads of the factorisation (0: all cores, at most 16)
 };
 
 struct IldlResult {

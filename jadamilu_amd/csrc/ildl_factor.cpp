@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace jdhip {
 
@@ -429,42 +430,74 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     SymCsr &S = out.S;
     S.n = nC;
     S.ptr.assign(nC + 1, 0);
-    std::vector<double> sv(nC, 0.0);
-    std::vector<int> smark(nC, -1), slist;
     const double sdrop = prm.droptol;
-    std::vector<std::pair<int, double>> srow;
-    for (int c = 0; c < nC; ++c) {
-        slist.clear();
-        int k = cnode[c];
-        auto sacc = [&](int j, double v) {
-            if (smark[j] != c) {
-                smark[j] = c;
-                sv[j] = v;
-                slist.push_back(j);
-            } else {
-                sv[j] += v;
+    // rows of S are independent: threads take contiguous row ranges, each
+    // with its own sparse accumulator, and the pieces are concatenated in
+    // row order (the result does not depend on the thread count)
+    struct Piece {
+        std::vector<int> cnt, idx;
+        std::vector<double> val;
+    };
+    const int nth = std::max(1, std::min(prm.threads, std::max(1, nC / 4096)));
+    std::vector<Piece> pieces(nth);
+    auto schur_rows = [&](int tid) {
+        const int c0 = (int)((long)nC * tid / nth), c1 = (int)((long)nC * (tid + 1) / nth);
+        Piece &P = pieces[tid];
+        P.cnt.assign(c1 - c0, 0);
+        std::vector<double> sv(nC, 0.0);
+        std::vector<int> smark(nC, -1), slist;
+        std::vector<std::pair<int, double>> srow;
+        for (int c = c0; c < c1; ++c) {
+            slist.clear();
+            int k = cnode[c];
+            auto sacc = [&](int j, double v) {
+                if (smark[j] != c) {
+                    smark[j] = c;
+                    sv[j] = v;
+                    slist.push_back(j);
+                } else {
+                    sv[j] += v;
+                }
+            };
+            for (int t = ap[k]; t < ap[k + 1]; ++t) {
+                int j = ai[t];
+                if (status[j] == 2) sacc(cpos[j], (double)av[t]);
             }
-        };
-        for (int t = ap[k]; t < ap[k + 1]; ++t) {
-            int j = ai[t];
-            if (status[j] == 2) sacc(cpos[j], (double)av[t]);
+            sacc(c, 0.0);
+            for (int t = L.lc_ptr[c]; t < L.lc_ptr[c + 1]; ++t) {
+                int b = L.lc_idx[t];
+                double coef = (double)L.lc_val[t] * (double)L.d[b];
+                for (int u = lcc_ptr[b]; u < lcc_ptr[b + 1]; ++u)
+                    sacc(lcc_idx[u], -coef * (double)lcc_val[u]);
+            }
+            srow.clear();
+            for (int j : slist)
+                if (j == c || std::fabs(sv[j]) > sdrop) srow.emplace_back(j, sv[j]);
+            std::sort(srow.begin(), srow.end(),
+                      [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
+            P.cnt[c - c0] = (int)srow.size();
+            for (auto &e : srow) {
+                P.idx.push_back(e.first);
+                P.val.push_back(e.second);
+            }
         }
-        sacc(c, 0.0);
-        for (int t = L.lc_ptr[c]; t < L.lc_ptr[c + 1]; ++t) {
-            int b = L.lc_idx[t];
-            double coef = (double)L.lc_val[t] * (double)L.d[b];
-            for (int u = lcc_ptr[b]; u < lcc_ptr[b + 1]; ++u)
-                sacc(lcc_idx[u], -coef * (double)lcc_val[u]);
-        }
-        srow.clear();
-        for (int j : slist)
-            if (j == c || std::fabs(sv[j]) > sdrop) srow.emplace_back(j, sv[j]);
-        std::sort(srow.begin(), srow.end(),
-                  [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
-        S.ptr[c + 1] = S.ptr[c] + (int)srow.size();
-        for (auto &e : srow) {
-            S.idx.push_back(e.first);
-            S.val.push_back(e.second);
+    };
+    if (nth == 1) {
+        schur_rows(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t) pool.emplace_back(schur_rows, t);
+        for (auto &th : pool) th.join();
+    }
+    {
+        int c = 0;
+        for (auto &P : pieces) {
+            for (int v : P.cnt) {
+                S.ptr[c + 1] = S.ptr[c] + v;
+                ++c;
+            }
+            S.idx.insert(S.idx.end(), P.idx.begin(), P.idx.end());
+            S.val.insert(S.val.end(), P.val.begin(), P.val.end());
         }
     }
     // symmetrise the pattern and values: S := (S + S^T) / 2 on the union

@@ -57,6 +57,7 @@ struct IldlParams {
     int max_levels = 40;
     bool spd = false;        // convert pivots to |d| (ISEARCH <= 1)
     int ordering = 2;        // 0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum degree
+    int threads = 1;         // host threads for the Schur complements
 };
 
 struct IldlStats {
