@@ -434,8 +434,8 @@ class IldlImpl : public IldlImplBase {
             }
         }
         if (kg > 0) {
-            // 2) per level (largest first), halve the count while the whole
-            //    apply stays within tol
+            // 2) per level (largest first), lower the count one by one while
+            //    the whole apply stays within tol
             std::vector<int> order(lv_.size());
             for (size_t l = 0; l < lv_.size(); ++l) order[l] = (int)l;
             std::stable_sort(order.begin(), order.end(),
@@ -443,7 +443,7 @@ class IldlImpl : public IldlImplBase {
             for (int l : order) {
                 while (lv_[l].k > 1) {
                     const int keep = lv_[l].k;
-                    lv_[l].k = keep / 2;
+                    lv_[l].k = keep > 8 ? keep * 3 / 4 : keep - 1;
                     if (!close()) {
                         lv_[l].k = keep;
                         break;
