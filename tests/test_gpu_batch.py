@@ -46,14 +46,15 @@ def test_batch_matches_dense_and_single_path(single):
         np.testing.assert_allclose(r.eigs, w[idx], rtol=1e-10, atol=1e-12)
         assert set(idx) <= set(np.argsort(np.abs(w))[:6].tolist())
         # relative residual ||A x - lambda x|| / (||A||_1 ||x||) <= TOL; the
-        # absolute one may exceed TOL slightly after the final Rayleigh-Ritz
-        # step (as in the reference, JDaux.F:2317-2372)
+        # absolute one may exceed TOL: RES is the true residual recomputed at
+        # termination, after convergence was declared on the updated one (as
+        # in the reference, JDaux.F:2317-2372)
         anorm = np.abs(A).sum(axis=0).max()
         assert (r.res <= 1e-10 * anorm).all()
         for k in range(r.neig):
             x = r.x[:, k]
             rk = np.linalg.norm(A @ x - r.eigs[k] * x)
-            assert rk <= 1e-10 * anorm * np.linalg.norm(x) and rk <= 1e-9, (k, rk)
+            assert rk <= 1e-10 * anorm * np.linalg.norm(x), (k, rk)
     # per-instance parity with the single-problem entry point
     from jadamilu_amd.jd import Context, lx_optimal
     for pr, r in list(zip(probs, out))[:3]:
