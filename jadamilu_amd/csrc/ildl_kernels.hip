@@ -1,8 +1,9 @@
 // ildl_kernels.hip -- gfx950 kernels of the multilevel ILDL apply
-// (see ildl_kernels.hpp).  Rows of the incomplete factor hold a handful of
-// entries, so a row is served by a group of 4 lanes (16 rows per wave64);
-// every loop is grid-stride and every read of a vector entry is a gather
-// through the CSR column index, coalesced along the row.
+// (see ildl_kernels.hpp).  Products with short rows run one lane per row
+// on SELL-C-sigma copies (every load of a wave contiguous); long rows run
+// on CSR with lane groups of 4/16/64 and a butterfly reduction; the exact
+// level-scheduled solve uses 4-lane groups per row of a level.  Every loop
+// is grid-stride; vector entries are gathered through the column index.
 #include "ildl_kernels.hpp"
 
 #include <algorithm>
@@ -68,24 +69,15 @@ __global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows,
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kB) void k_jacobi(int nrows, const int *__restrict__ ptr,
-                                               const int *__restrict__ idx,
-                                               const T *__restrict__ val,
-                                               const T *__restrict__ b,
-                                               const T *__restrict__ xin, T *__restrict__ xout) {
-    const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
-    for (long r = (long)blockIdx.x * (kB / kLpr) + g; r < nrows;
-         r += (long)gridDim.x * (kB / kLpr)) {
-        T s = T(0);
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += val[k] * xin[idx[k]];
-#pragma unroll
-        for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
-        if (l == 0) xout[r] = b[r] - s;
-    }
+// Loads of the matrix stream; NT = nontemporal, so the single-use values and
+// columns do not push the reused x entries out of L2.
+template <bool NT, typename X>
+__device__ __forceinline__ X ld(const X *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
-template <typename T>
+template <typename T, bool NT>
 __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__restrict__ vptr,
                                                     const int *__restrict__ wid,
                                                     const int *__restrict__ rowid,
@@ -106,10 +98,10 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
         T s0 = T(0), s1 = T(0);
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += v[k * 64] * xin[c[k * 64]];
-            s1 += v[(k + 1) * 64] * xin[c[(k + 1) * 64]];
+            s0 += ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
+            s1 += ld<NT>(v + (k + 1) * 64) * xin[ld<NT>(c + (k + 1) * 64)];
         }
-        if (k < w) s0 += v[k * 64] * xin[c[k * 64]];
+        if (k < w) s0 += ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
         xout[r] = b[r] - (s0 + s1);
     }
 }
@@ -175,22 +167,6 @@ __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict
 }
 
 template <typename T>
-__global__ __launch_bounds__(kB) void k_spmv_sub(int nrows, const int *__restrict__ ptr,
-                                                 const int *__restrict__ idx,
-                                                 const T *__restrict__ val,
-                                                 const T *__restrict__ x, T *__restrict__ y) {
-    const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
-    for (long r = (long)blockIdx.x * (kB / kLpr) + g; r < nrows;
-         r += (long)gridDim.x * (kB / kLpr)) {
-        T s = T(0);
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += val[k] * x[idx[k]];
-#pragma unroll
-        for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
-        if (l == 0) y[r] -= s;
-    }
-}
-
-template <typename T>
 __global__ __launch_bounds__(kB) void k_dscale(int n, const T *__restrict__ dinv, T *x) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
         x[i] *= dinv[i];
@@ -229,18 +205,16 @@ void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, 
     k_trsv_level<T><<<blocks_for(count, kB / kLpr), kB, 0, s>>>(rows, count, ptr, idx, val, x);
 }
 template <typename T>
-void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
-                       const T *b, const T *xin, T *xout) {
-    if (nrows <= 0) return;
-    k_jacobi<T><<<blocks_for(nrows, kB / kLpr), kB, 0, s>>>(nrows, ptr, idx, val, b, xin, xout);
-}
-template <typename T>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                             const int *rowid, const T *val, const int *col, const T *b,
                             const T *xin, T *xout, int xcd) {
     if (nslices <= 0) return;
-    k_jacobi_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
-        nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd);
+    if (xcd & 2)
+        k_jacobi_sell<T, true><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
+            nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd & 1);
+    else
+        k_jacobi_sell<T, false><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
+            nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd & 1);
 }
 template <typename T>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
@@ -279,12 +253,6 @@ void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *
                                                                          y);
 }
 template <typename T>
-void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
-                   const T *x, T *y) {
-    if (nrows <= 0) return;
-    k_spmv_sub<T><<<blocks_for(nrows, kB / kLpr), kB, 0, s>>>(nrows, ptr, idx, val, x, y);
-}
-template <typename T>
 void ildl_dscale(hipStream_t s, int n, const T *dinv, T *x) {
     if (n <= 0) return;
     k_dscale<T><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
@@ -298,8 +266,6 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
 #define INST(T)                                                                              \
     template void ildl_trsv_level<T>(hipStream_t, const int *, int, const int *, const int *, \
                                      const T *, T *);                                        \
-    template void ildl_jacobi_sweep<T>(hipStream_t, int, const int *, const int *, const T *, \
-                                       const T *, const T *, T *);                           \
     template void ildl_jacobi_sweep_sell<T>(hipStream_t, int, const long *, const int *,      \
                                             const int *, const T *, const int *, const T *,  \
                                             const T *, T *, int);                            \
@@ -309,8 +275,6 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
                                      const T *, const T *, const T *, T *);                  \
     template void ildl_sub_csr<T>(hipStream_t, int, int, const int *, const int *, const T *, \
                                   const T *, T *);                                           \
-    template void ildl_spmv_sub<T>(hipStream_t, int, const int *, const int *, const T *,     \
-                                   const T *, T *);                                          \
     template void ildl_dscale<T>(hipStream_t, int, const T *, T *);                          \
     template void ildl_dense_gemv<T>(hipStream_t, int, const T *, const T *, T *);
 INST(double)

@@ -21,15 +21,12 @@ void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const T *scal, 
 template <typename T>
 void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
                      const T *val, T *x);
-// xout = b - M xin over nrows rows (one Jacobi sweep of the unit triangle)
-template <typename T>
-void ildl_jacobi_sweep(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
-                       const T *b, const T *xin, T *xout);
-// The same sweep on a SELL-C-sigma copy of the triangle: slices of 64 rows
-// (one wave64, one lane per row) taken from windows of rows sorted by
-// length, entries of a slice stored column-major (value and column at
+// One Jacobi sweep of the unit triangle, xout = b - M xin, on a SELL-C-sigma
+// copy of M: slices of 64 rows (one wave64, one lane per row) taken from
+// windows of rows sorted by length, entries of a slice stored column-major (value and column at
 // vptr[s] + k*64 + lane, k < wid[s]); rowid[s*64 + lane] is the row a lane
-// owns (-1 for padding lanes).  xcd != 0: XCD-contiguous block order.
+// owns (-1 for padding lanes).  xcd bit 0: XCD-contiguous block order; bit 1
+// (sweeps): nontemporal loads of the matrix stream.
 template <typename T>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                             const int *rowid, const T *val, const int *col, const T *b,
@@ -46,10 +43,6 @@ template <typename T>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
                    const int *rowid, const T *val, const int *col, const T *x, T *y,
                    int xcd);
-// y[r] -= sum_j M(r, j) x[j] for r < nrows, then optional y[r] *= dscale[r]
-template <typename T>
-void ildl_spmv_sub(hipStream_t s, int nrows, const int *ptr, const int *idx, const T *val,
-                   const T *x, T *y);
 template <typename T>
 void ildl_dscale(hipStream_t s, int n, const T *dinv, T *x);
 // y = Minv x, Minv m x m row-major
