@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--inner64", action="store_true",
                     help="with --single: keep the inner solver in FP64 (FP32 factor only, the "
                          "reference's ICNTL(5) semantics)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the preconditioner apply from captured HIP graphs")
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts of the batched workload")
     ap.add_argument("--kernels-only", action="store_true",
@@ -340,6 +342,7 @@ def main():
     icntl = (0, -1, 0, 0, single)  # timed solves: keep the resident factor
     ctx.set_option("ildl_sweeps", args.sweeps)
     ctx.set_option("inner_fp32", 0 if args.inner64 else 1)
+    ctx.set_option("ildl_graph", 1 if args.graph else 0)
 
     def one_solve(icntl2):
         X[:n].copy_(x0_dev)
