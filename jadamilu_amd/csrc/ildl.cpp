@@ -96,15 +96,20 @@ struct SellHost {
     std::vector<T> val;
 };
 
+// skip_empty: rows without entries get no lane (products y -= M x, where such
+// rows are no-ops; a sweep must still copy b into them, so it keeps them).
 template <typename T>
 SellHost<T> build_sell(int nrows, const std::vector<int> &ptr, const std::vector<int> &idx,
-                       const std::vector<T> &val, int sigma) {
+                       const std::vector<T> &val, int sigma, bool skip_empty = false) {
     SellHost<T> h;
     std::vector<int> order(nrows);
     sigma = std::max(sigma, 64);
-    for (int w0 = 0; w0 < nrows; w0 += sigma) {
-        const int w1 = std::min(nrows, w0 + sigma);
-        for (int r = w0; r < w1; ++r) order[r] = r;
+    for (int r0 = 0; r0 < nrows; r0 += sigma) {
+        const int r1 = std::min(nrows, r0 + sigma);
+        const int w0 = r0;
+        int w1 = r0;
+        for (int r = r0; r < r1; ++r)
+            if (!skip_empty || ptr[r + 1] > ptr[r]) order[w1++] = r;
         if (sigma > 64)
             std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
                 return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b];
@@ -280,14 +285,14 @@ class IldlImpl : public IldlImplBase {
                 for (int &r : rows) r = qinv[r];
                 D.bwd_rows = track(dev_upload(rows));
             }
-            D.sc = upload_sell(build_sell(nC, cp, ci, cv, sigma_));
+            D.sc = upload_sell(build_sell(nC, cp, ci, cv, sigma_, true));
             D.lc_ptr = track(dev_upload(cp));
             D.lc_idx = track(dev_upload(ci));
             D.lc_val = track(dev_upload(cv));
             D.wc = pick_lanes((long)ci.size(), nC);
             D.wf = D.wb = pick_lanes((long)bi.size(), nB);
             transpose(nC, nB, cp, ci, cv, tp, ti, tv);
-            D.sct = upload_sell(build_sell(nB, tp, ti, tv, sigma_));
+            D.sct = upload_sell(build_sell(nB, tp, ti, tv, sigma_, true));
             D.lct_ptr = track(dev_upload(tp));
             D.lct_idx = track(dev_upload(ti));
             D.lct_val = track(dev_upload(tv));

@@ -34,6 +34,25 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const T *__restrict__ V, lo
     }
 }
 
+// finish of a multi-column reduction: block c sums column c of the partials
+// (columns are independent, so they are summed in parallel)
+__global__ __launch_bounds__(kBlock) void k_finish_columns(const double *__restrict__ partials,
+                                                          int nparts, double *out) {
+    __shared__ double red[kBlock / 64];
+    const int c = blockIdx.x, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) s += partials[(long)c * nparts + b];
+    s = wave_sum(s);
+    if (lane == 0) red[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        out[c] = t;
+    }
+}
+
 template <typename T>
 void multidot(hipStream_t s, const RedBuf &rb, const T *V, long ldv, int jv, const T *w, long n,
               double *out) {
@@ -43,7 +62,7 @@ void multidot(hipStream_t s, const RedBuf &rb, const T *V, long ldv, int jv, con
 #define JDHIP_MD(NC)                                                                    \
     do {                                                                                \
         k_multidot<T, NC><<<g, kBlock, 0, s>>>(V, ldv, jv, w, n, P, 0, out);            \
-        k_multidot<T, NC><<<1, kBlock, 0, s>>>(V, ldv, jv, w, 0, P, g, out);            \
+        k_finish_columns<<<jv, kBlock, 0, s>>>(P, g, out);                               \
     } while (0)
     if (jv <= 1) JDHIP_MD(1);
     else if (jv <= 2) JDHIP_MD(2);
