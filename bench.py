@@ -284,12 +284,14 @@ def main():
                     help="ILDL triangular solves: 0 exact level-scheduled, k Jacobi sweeps, "
                          "-1 tuned at set-up")
     ap.add_argument("--single", action="store_true",
-                    help="mixed path: FP32 ILDL factor (ICNTL(5)=1) and FP32 inner solver")
-    ap.add_argument("--inner64", action="store_true",
-                    help="with --single: keep the inner solver in FP64 (FP32 factor only, the "
-                         "reference's ICNTL(5) semantics)")
+                    help="mixed path: FP32 ILDL factor (ICNTL(5)=1), FP64 inner and outer "
+                         "iteration")
+    ap.add_argument("--inner32", action="store_true",
+                    help="with --single: run the inner solver in FP32 as well")
     ap.add_argument("--graph", action="store_true",
                     help="replay the preconditioner apply from captured HIP graphs")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="extra jdhip_set_option(NAME, VALUE), repeatable")
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts of the batched workload")
     ap.add_argument("--kernels-only", action="store_true",
@@ -341,8 +343,11 @@ def main():
     single = 1 if args.single else 0
     icntl = (0, -1, 0, 0, single)  # timed solves: keep the resident factor
     ctx.set_option("ildl_sweeps", args.sweeps)
-    ctx.set_option("inner_fp32", 0 if args.inner64 else 1)
+    ctx.set_option("inner_fp32", 1 if args.inner32 else 0)
     ctx.set_option("ildl_graph", 1 if args.graph else 0)
+    for kv in args.set:
+        name, value = kv.split("=", 1)
+        ctx.set_option(name, float(value))
 
     def one_solve(icntl2):
         X[:n].copy_(x0_dev)
@@ -493,8 +498,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 factor + f64 inner/outer" if args.inner64 else
-                  "f32 inner/factor + f64 outer") if args.single else "f64",
+        "dtype": ("f32 inner/factor + f64 outer" if args.inner32 else
+                  "f32 factor + f64 inner/outer") if args.single else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz_upper": nnz_u, "neig": neig,
                    "tol": args.tol, "madspace": madspace, "isearch": isearch, "sigma": sigma,
@@ -503,6 +508,7 @@ def main():
                                f"({'FP32' if args.single else 'FP64'} factor, "
                                f"triangular solves: {tri_mode})"),
                    "icntl": list(icntl),
+                   "options": dict(kv.split("=", 1) for kv in args.set),
                    "start": "NINIT=1, numpy PCG64 seed 0 uniform[0,1)"},
         "roofline": roofline,
         "roofline_spmv": roof_spmv,

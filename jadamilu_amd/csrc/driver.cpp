@@ -163,7 +163,9 @@ struct Solver {
     void precond_request_f(float *z, const float *r);
     void adapt();
     float *d_diagf = nullptr;
-    bool inner_fp32 = true;  // mixed path: FP32 inner solver when ICNTL(5) != 0
+    // ICNTL(5) != 0: FP32 factor; with inner_fp32 also an FP32 inner solver
+    bool inner_fp32 = false;
+    bool inner_spmv64 = true;  // FP32 inner solver: SpMV reads the FP64 values
 
     void pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, double *res,
                   double *x_dev, long lx, int *neig_, double sigma_, int isearch_, int ninit,
@@ -179,6 +181,7 @@ void SolverOps::matvec(const double *x, double *y, const double *eta_dev, double
 }
 void SolverOps::precond(JdEngine &, double *z, const double *r) { S->precond_request(z, r); }
 void SolverOps::matvec_f(const float *x, float *y, double eta, const float *u, double *udot) {
+    S->A.f_vals64 = S->inner_spmv64;
     spmv<float>(S->ctx.stream, S->ctx.rb, S->A, x, y, eta, u, udot);
 }
 void SolverOps::precond_f(JdEngine &, float *z, const float *r) { S->precond_request_f(z, r); }
@@ -950,6 +953,7 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_dense_max") S.ildl_dense_max = (int)value;
     else if (k == "spmv_lanes") S.A.lpr = (int)value;
     else if (k == "inner_fp32") S.inner_fp32 = value != 0.0;
+    else if (k == "inner_spmv64") S.inner_spmv64 = value != 0.0;
     else if (k == "ildl_storage") S.ildl_storage = (int)value;
     else if (k == "ildl_xcd") S.ildl_xcd = (int)value;
     else if (k == "ildl_sigma") S.ildl_sigma = (int)value;

@@ -821,6 +821,7 @@ L40i:  // inner iteration fails (JDaux.F:3255-3266)
                 hipMemcpyAsync(D(xcg), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
     }
 L34:
+    f32_last_ = f32_;
     if (f32_) {
         convert(s, D(xcg), fbuf_ + 4L * n, n);  // correction back to FP64
         f32_ = false;
@@ -830,7 +831,10 @@ L34:
     conv = false;
     flagcg = false;
     rcum = rcum + std::log(nreig / resid);
-    if (resid >= tol) goto L50;
+    // The FP32 inner solver's recurrences cannot certify a residual below
+    // TOL (their rounding floor lies above it): its correction always goes
+    // through the outer step, whose FP64 residual decides convergence.
+    if (resid >= tol || f32_last_) goto L50;
     // convergence reached during the inner solve
     axpy(s, D(uu), -1.0, D(xcg), n);
     repgs(n, kxp, *neig, uu, ycoef, 0);

@@ -115,6 +115,7 @@ class JdEngine {
     float *fbuf_ = nullptr;
     long fbuf_len_ = 0;
     bool f32_ = false;
+    bool f32_last_ = false;  // the last inner solve ran in FP32
     void do_matvec(const double *xin, double *yout) {
         ops_->matvec(xin, yout, nullptr, 0.0, nullptr, nullptr);
     }

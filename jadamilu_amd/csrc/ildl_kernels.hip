@@ -4,6 +4,10 @@
 // on CSR with lane groups of 4/16/64 and a butterfly reduction; the exact
 // level-scheduled solve uses 4-lane groups per row of a level.  Every loop
 // is grid-stride; vector entries are gathered through the column index.
+// Sums are accumulated in double also for the FP32 factor (stored and
+// streamed in float): the substitutions cancel large terms, and float
+// accumulation measurably slows the outer iteration on indefinite
+// matrices, while the extra arithmetic is free in these HBM-bound loops.
 #include "ildl_kernels.hpp"
 
 #include <algorithm>
@@ -61,11 +65,11 @@ __global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows,
     for (long t = (long)blockIdx.x * (kB / kLpr) + g; t < count;
          t += (long)gridDim.x * (kB / kLpr)) {
         const int r = rows[t];
-        T s = T(0);
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += val[k] * x[idx[k]];
+        double s = 0.0;
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += (double)val[k] * (double)x[idx[k]];
 #pragma unroll
         for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
-        if (l == 0) x[r] -= s;
+        if (l == 0) x[r] = (T)((double)x[r] - s);
     }
 }
 
@@ -95,14 +99,14 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
         const int w = wid[sl];
         const T *v = val + vptr[sl] + lane;
         const int *c = col + vptr[sl] + lane;
-        T s0 = T(0), s1 = T(0);
+        double s0 = 0.0, s1 = 0.0;
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
-            s1 += ld<NT>(v + (k + 1) * 64) * xin[ld<NT>(c + (k + 1) * 64)];
+            s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
+            s1 += (double)ld<NT>(v + (k + 1) * 64) * (double)xin[ld<NT>(c + (k + 1) * 64)];
         }
-        if (k < w) s0 += ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
-        xout[r] = b[r] - (s0 + s1);
+        if (k < w) s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
+        xout[r] = (T)((double)b[r] - (s0 + s1));
     }
 }
 
@@ -122,14 +126,14 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
         const int w = wid[sl];
         const T *v = val + vptr[sl] + lane;
         const int *c = col + vptr[sl] + lane;
-        T s0 = T(0), s1 = T(0);
+        double s0 = 0.0, s1 = 0.0;
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += v[k * 64] * x[c[k * 64]];
-            s1 += v[(k + 1) * 64] * x[c[(k + 1) * 64]];
+            s0 += (double)v[k * 64] * (double)x[c[k * 64]];
+            s1 += (double)v[(k + 1) * 64] * (double)x[c[(k + 1) * 64]];
         }
-        if (k < w) s0 += v[k * 64] * x[c[k * 64]];
-        y[r] -= s0 + s1;
+        if (k < w) s0 += (double)v[k * 64] * (double)x[c[k * 64]];
+        y[r] = (T)((double)y[r] - (s0 + s1));
     }
 }
 
@@ -143,11 +147,11 @@ __global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restr
                                                    const T *xin, T *xout) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
-        T s = T(0);
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += val[k] * xin[idx[k]];
+        double s = 0.0;
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)xin[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
-        if (l == 0) xout[r] = b[r] - s;
+        if (l == 0) xout[r] = (T)((double)b[r] - s);
     }
 }
 
@@ -158,11 +162,11 @@ __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict
                                                 const T *__restrict__ x, T *__restrict__ y) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
-        T s = T(0);
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += val[k] * x[idx[k]];
+        double s = 0.0;
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
-        if (l == 0) y[r] -= s;
+        if (l == 0) y[r] = (T)((double)y[r] - s);
     }
 }
 
@@ -179,11 +183,11 @@ __global__ __launch_bounds__(kB) void k_dense_gemv(int m, const T *__restrict__ 
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (long r = (long)blockIdx.x * (kB / 64) + w; r < m; r += (long)gridDim.x * (kB / 64)) {
         const T *row = Minv + r * (long)m;
-        T s = T(0);
-        for (int j = lane; j < m; j += 64) s += row[j] * x[j];
+        double s = 0.0;
+        for (int j = lane; j < m; j += 64) s += (double)row[j] * (double)x[j];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) y[r] = s;
+        if (lane == 0) y[r] = (T)s;
     }
 }
 

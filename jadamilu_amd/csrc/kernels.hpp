@@ -82,6 +82,7 @@ struct CsrDev {
     char *s_idx = nullptr;
     long s_entries = 0;       // padded entries
     long s_idx_bytes = 0;
+    bool f_vals64 = false;    // FP32 vectors read the double values
 };
 // y = A x (T = double: A.val, T = float: A.valf); if eta != 0: y -= eta x;
 // if u: *udot_out = u.y (double)

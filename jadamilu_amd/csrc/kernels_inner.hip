@@ -77,27 +77,29 @@ void multidot(hipStream_t s, const RedBuf &rb, const T *V, long ldv, int jv, con
 // ---------------------------------------------------------------------------
 // SpMV on the device-built full CSR; LPR lanes per row; optional epilogue
 //   y_i -= eta x_i   and   udot += u_i y_i   (u.y in double).
-template <typename T, int LPR, bool ETA, bool UDOT>
+// Rows are accumulated in double also for FP32 storage: near an eigenvalue
+// of A - eta I the row sums cancel to a tiny fraction of their terms, and a
+// float accumulator would leave errors of the size of that result.
+template <typename T, typename TA, int LPR, bool ETA, bool UDOT>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ rowptr,
                                                  const int *__restrict__ col,
-                                                 const T *__restrict__ val,
+                                                 const TA *__restrict__ val,
                                                  const T *__restrict__ x, T *__restrict__ y,
                                                  double eta_host, const T *__restrict__ u,
                                                  double *partials, int nparts,
                                                  double *udot_out) {
     constexpr int GPB = kBlock / LPR;
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    const T eta = (T)eta_host;
     double acc[1] = {0.0};
     for (long row = (long)blockIdx.x * GPB + g; row < n; row += (long)gridDim.x * GPB) {
         const int b = rowptr[row], e = rowptr[row + 1];
-        T s = T(0);
-        for (int k = b + l; k < e; k += LPR) s += val[k] * x[col[k]];
+        double s = 0.0;
+        for (int k = b + l; k < e; k += LPR) s += (double)val[k] * (double)x[col[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
         if (l == 0) {
-            T yi = s;
-            if (ETA) yi += (-eta) * x[row];
+            if (ETA) s -= eta_host * (double)x[row];
+            const T yi = (T)s;
             y[row] = yi;
             if (UDOT) acc[0] += (double)u[row] * (double)yi;
         }
@@ -109,9 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
     }
 }
 
-template <typename T, int LPR>
+template <typename T, typename TA, int LPR>
 static void spmv_lpr(hipStream_t s, const RedBuf &rb, int n, const int *rp, const int *ci,
-                     const T *val, const T *x, T *y, double eta, const T *u, double *udot) {
+                     const TA *val, const T *x, T *y, double eta, const T *u, double *udot) {
     constexpr int GPB = kBlock / LPR;
     long rows_blocks = ((long)n + GPB - 1) / GPB;
     const bool has_eta = eta != 0.0;
@@ -119,64 +121,64 @@ static void spmv_lpr(hipStream_t s, const RedBuf &rb, int n, const int *rp, cons
     if (u) {
         int g = (int)std::min<long>(rows_blocks, kMaxRedBlocks);
         if (has_eta) {
-            k_spmv<T, LPR, true, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+            k_spmv<T, TA, LPR, true, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
                                                            udot);
-            k_spmv<T, LPR, true, true><<<1, kBlock, 0, s>>>(0, rp, ci, val, x, y, eta, u, P, g,
+            k_spmv<T, TA, LPR, true, true><<<1, kBlock, 0, s>>>(0, rp, ci, val, x, y, eta, u, P, g,
                                                            udot);
         } else {
-            k_spmv<T, LPR, false, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+            k_spmv<T, TA, LPR, false, true><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
                                                             udot);
-            k_spmv<T, LPR, false, true><<<1, kBlock, 0, s>>>(0, rp, ci, val, x, y, eta, u, P, g,
+            k_spmv<T, TA, LPR, false, true><<<1, kBlock, 0, s>>>(0, rp, ci, val, x, y, eta, u, P, g,
                                                             udot);
         }
     } else {
         int g = (int)std::min<long>(rows_blocks, 8192);
         if (has_eta)
-            k_spmv<T, LPR, true, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+            k_spmv<T, TA, LPR, true, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
                                                             udot);
         else
-            k_spmv<T, LPR, false, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
+            k_spmv<T, TA, LPR, false, false><<<g, kBlock, 0, s>>>(n, rp, ci, val, x, y, eta, u, P, 0,
                                                              udot);
     }
 }
 
 // Sliced ELL: one wave64 per slice of 64 rows, lane = row within the slice.
-template <typename T, bool ETA, bool UDOT>
+template <typename T, typename TA, bool ETA, bool UDOT>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     int n, int nslices, const long *__restrict__ vptr, const long *__restrict__ iptr,
-    const int *__restrict__ wm, const T *__restrict__ val, const char *__restrict__ idx,
+    const int *__restrict__ wm, const TA *__restrict__ val, const char *__restrict__ idx,
     const T *__restrict__ x, T *__restrict__ y, double eta_host, const T *__restrict__ u,
     double *partials, int nparts, double *udot_out) {
     constexpr int WPB = kBlock / 64;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const T eta = (T)eta_host;
     double acc[1] = {0.0};
     for (long sl = (long)blockIdx.x * WPB + wv; sl < nslices; sl += (long)gridDim.x * WPB) {
         const long row = sl * 64 + lane;
         if (row >= n) continue;
         const int code = wm[sl];
         const int w = code >> 1;
-        const T *v = val + vptr[sl] + lane;
-        T s0 = T(0), s1 = T(0);
+        const TA *v = val + vptr[sl] + lane;
+        double s0 = 0.0, s1 = 0.0;
         if (code & 1) {
             const short *ci = (const short *)(idx + iptr[sl]) + lane;
             int k = 0;
             for (; k + 1 < w; k += 2) {
-                s0 += v[k * 64] * x[row + ci[k * 64]];
-                s1 += v[(k + 1) * 64] * x[row + ci[(k + 1) * 64]];
+                s0 += (double)v[k * 64] * (double)x[row + ci[k * 64]];
+                s1 += (double)v[(k + 1) * 64] * (double)x[row + ci[(k + 1) * 64]];
             }
-            if (k < w) s0 += v[k * 64] * x[row + ci[k * 64]];
+            if (k < w) s0 += (double)v[k * 64] * (double)x[row + ci[k * 64]];
         } else {
             const int *ci = (const int *)(idx + iptr[sl]) + lane;
             int k = 0;
             for (; k + 1 < w; k += 2) {
-                s0 += v[k * 64] * x[ci[k * 64]];
-                s1 += v[(k + 1) * 64] * x[ci[(k + 1) * 64]];
+                s0 += (double)v[k * 64] * (double)x[ci[k * 64]];
+                s1 += (double)v[(k + 1) * 64] * (double)x[ci[(k + 1) * 64]];
             }
-            if (k < w) s0 += v[k * 64] * x[ci[k * 64]];
+            if (k < w) s0 += (double)v[k * 64] * (double)x[ci[k * 64]];
         }
-        T yi = s0 + s1;
-        if (ETA) yi += (-eta) * x[row];
+        double ys = s0 + s1;
+        if (ETA) ys -= eta_host * (double)x[row];
+        const T yi = (T)ys;
         y[row] = yi;
         if (UDOT) acc[0] += (double)u[row] * (double)yi;
     }
@@ -187,8 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     }
 }
 
-template <typename T>
-static void spmv_sell(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T *val,
+template <typename T, typename TA>
+static void spmv_sell(hipStream_t s, const RedBuf &rb, const CsrDev &A, const TA *val,
                       const T *x, T *y, double eta, const T *u, double *udot) {
     constexpr int WPB = kBlock / 64;
     const long blocks = ((long)A.nslices + WPB - 1) / WPB;
@@ -198,54 +200,60 @@ static void spmv_sell(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T 
         // one partial sum per block: the full grid fits the scratch
         int g = (int)std::min<long>(blocks, kMaxPartials);
         if (has_eta) {
-            k_spmv_sell<T, true, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr, A.s_iptr,
+            k_spmv_sell<T, TA, true, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr, A.s_iptr,
                                                           A.s_wm, val, A.s_idx, x, y, eta, u,
                                                           P, 0, udot);
-            k_spmv_sell<T, true, true><<<1, kBlock, 0, s>>>(A.n, 0, A.s_vptr, A.s_iptr,
+            k_spmv_sell<T, TA, true, true><<<1, kBlock, 0, s>>>(A.n, 0, A.s_vptr, A.s_iptr,
                                                           A.s_wm, val, A.s_idx, x, y, eta, u,
                                                           P, g, udot);
         } else {
-            k_spmv_sell<T, false, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
+            k_spmv_sell<T, TA, false, true><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
                                                            A.s_iptr, A.s_wm, val, A.s_idx, x, y,
                                                            eta, u, P, 0, udot);
-            k_spmv_sell<T, false, true><<<1, kBlock, 0, s>>>(A.n, 0, A.s_vptr, A.s_iptr,
+            k_spmv_sell<T, TA, false, true><<<1, kBlock, 0, s>>>(A.n, 0, A.s_vptr, A.s_iptr,
                                                            A.s_wm, val, A.s_idx, x, y, eta, u,
                                                            P, g, udot);
         }
     } else {
         int g = (int)std::min<long>(blocks, 1L << 20);
         if (has_eta)
-            k_spmv_sell<T, true, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
+            k_spmv_sell<T, TA, true, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
                                                            A.s_iptr, A.s_wm, val, A.s_idx, x, y,
                                                            eta, u, P, 0, udot);
         else
-            k_spmv_sell<T, false, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
+            k_spmv_sell<T, TA, false, false><<<g, kBlock, 0, s>>>(A.n, A.nslices, A.s_vptr,
                                                             A.s_iptr, A.s_wm, val, A.s_idx, x,
                                                             y, eta, u, P, 0, udot);
+    }
+}
+
+template <typename T, typename TA>
+static void spmv_vals(hipStream_t s, const RedBuf &rb, const CsrDev &A, const TA *val,
+                      const TA *sv, const T *x, T *y, double eta, const T *u,
+                      double *udot_out) {
+    if (A.lpr == 0 && A.nslices > 0) {
+        spmv_sell<T, TA>(s, rb, A, sv, x, y, eta, u, udot_out);
+        return;
+    }
+    switch (A.lpr) {
+    case 1: spmv_lpr<T, TA, 1>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
+    case 2: spmv_lpr<T, TA, 2>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
+    case 4: spmv_lpr<T, TA, 4>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
+    case 16: spmv_lpr<T, TA, 16>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
+    case 32: spmv_lpr<T, TA, 32>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
+    default: spmv_lpr<T, TA, 8>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
     }
 }
 
 template <typename T>
 void spmv(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T *x, T *y, double eta,
           const T *u, double *udot_out) {
-    const T *val;
-    if constexpr (sizeof(T) == 8) val = (const T *)A.val;
-    else val = (const T *)A.valf;
-    if (A.lpr == 0 && A.nslices > 0) {
-        const T *sv;
-        if constexpr (sizeof(T) == 8) sv = (const T *)A.s_val;
-        else sv = (const T *)A.s_valf;
-        spmv_sell<T>(s, rb, A, sv, x, y, eta, u, udot_out);
-        return;
-    }
-    switch (A.lpr) {
-    case 1: spmv_lpr<T, 1>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
-    case 2: spmv_lpr<T, 2>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
-    case 4: spmv_lpr<T, 4>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
-    case 16: spmv_lpr<T, 16>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
-    case 32: spmv_lpr<T, 32>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
-    default: spmv_lpr<T, 8>(s, rb, A.n, A.rowptr, A.col, val, x, y, eta, u, udot_out); break;
-    }
+    if constexpr (sizeof(T) == 8)
+        spmv_vals<T, double>(s, rb, A, A.val, A.s_val, x, y, eta, u, udot_out);
+    else if (A.f_vals64)
+        spmv_vals<T, double>(s, rb, A, A.val, A.s_val, x, y, eta, u, udot_out);
+    else
+        spmv_vals<T, float>(s, rb, A, A.valf, A.s_valf, x, y, eta, u, udot_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -282,15 +290,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_project_rho(T *__restrict__ z,
                                                            const double *__restrict__ ycoef,
                                                            double *partials, int nparts) {
     if (sc->flag) return;
-    __shared__ T ys[NC];
-    if ((int)threadIdx.x < jq) ys[threadIdx.x] = (T)(-ycoef[threadIdx.x]);
+    // the projection cancels the large components of z along the basis:
+    // summed in double also for FP32 vectors
+    __shared__ double ys[NC];
+    if ((int)threadIdx.x < jq) ys[threadIdx.x] = -ycoef[threadIdx.x];
     __syncthreads();
     double acc[1] = {0.0};
     for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
-        T zi = z[i];
+        double zd = (double)z[i];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-            if (c < jq) zi += ys[c] * Q[c * n + i];
+            if (c < jq) zd += ys[c] * (double)Q[c * n + i];
+        const T zi = (T)zd;
         z[i] = zi;
         acc[0] += (double)r[i] * (double)zi;
     }
