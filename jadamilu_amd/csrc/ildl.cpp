@@ -162,7 +162,7 @@ template <typename T>
 struct DevLevel {
     int n = 0, nB = 0, nC = 0;
     int *perm = nullptr;
-    T *scal = nullptr, *dinv = nullptr;
+    double *scal = nullptr, *dinv = nullptr;  // per row, double on both paths
     int *lb_ptr = nullptr, *lb_idx = nullptr;
     T *lb_val = nullptr;
     int *lt_ptr = nullptr, *lt_idx = nullptr;
@@ -173,7 +173,7 @@ struct DevLevel {
     T *lct_val = nullptr;
     std::vector<int> fwd_ptr, bwd_ptr;  // host level pointers
     int *fwd_rows = nullptr, *bwd_rows = nullptr;
-    T *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;
+    double *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;  // work vectors
     long nnz_lb = 0, nnz_lc = 0;
     SellDev<T> sf, sb;  // sweep copies of L_B (forward) and L_B^T (backward)
     int k = 0;          // this level's triangular solves: 0 exact, k Jacobi sweeps
@@ -216,7 +216,7 @@ class IldlImpl : public IldlImplBase {
                 for (int i = 0; i < n; ++i) orig[i] = i;
             }
             std::vector<int> perm(n);
-            std::vector<T> scal(n);
+            std::vector<double> scal(n);
             for (int i = 0; i < n; ++i) perm[i] = cnew.empty() ? L.perm[i] : cnew[L.perm[i]];
             for (int j = 0; j < n; ++j) scal[cnew.empty() ? j : cnew[j]] = L.scal[j];
             std::vector<int> q(n), qinv(n);
@@ -236,8 +236,8 @@ class IldlImpl : public IldlImplBase {
             for (int t = 0; t < n; ++t) permN[t] = perm[q[t]];
             D.perm = track(dev_upload(permN));
             D.scal = track(dev_upload(scal));
-            std::vector<T> dinv(nB);
-            for (int t = 0; t < nB; ++t) dinv[t] = T(1) / L.d[q[t]];
+            std::vector<double> dinv(nB);
+            for (int t = 0; t < nB; ++t) dinv[t] = 1.0 / (double)L.d[q[t]];
             D.dinv = track(dev_upload(dinv));
             // L_B and L_CB in the storage numbering
             std::vector<int> bp(nB + 1, 0), bi, cp(nC + 1, 0), ci;
@@ -297,10 +297,10 @@ class IldlImpl : public IldlImplBase {
             D.lct_idx = track(dev_upload(ti));
             D.lct_val = track(dev_upload(tv));
             D.wct = pick_lanes((long)ti.size(), nB);
-            D.buf = track(dev_alloc<T>(n));
-            D.out = track(dev_alloc<T>(n));
-            D.t1 = track(dev_alloc<T>(std::max(nB, 1)));
-            D.t2 = track(dev_alloc<T>(std::max(nB, 1)));
+            D.buf = track(dev_alloc<double>(n));
+            D.out = track(dev_alloc<double>(n));
+            D.t1 = track(dev_alloc<double>(std::max(nB, 1)));
+            D.t2 = track(dev_alloc<double>(std::max(nB, 1)));
             D.nnz_lb = (long)L.lb_idx.size();
             D.nnz_lc = (long)L.lc_idx.size();
             res.fwd_levels.push_back((int)D.fwd_ptr.size() - 1);
@@ -326,8 +326,8 @@ class IldlImpl : public IldlImplBase {
                     mv[(size_t)a * m + b] = (T)f_.dinv[(size_t)oa * m + ob];
                 }
             dmat_ = track(dev_upload(mv));
-            dx_ = track(dev_alloc<T>(m));
-            dy_ = track(dev_alloc<T>(m));
+            dx_ = track(dev_alloc<double>(m));
+            dy_ = track(dev_alloc<double>(m));
         }
         for (auto &D : lv_) D.k = sweeps_;
         JDHIP_CHECK(hipDeviceSynchronize());
@@ -352,11 +352,11 @@ class IldlImpl : public IldlImplBase {
         // the other buffer (needs at least 2 sweeps; 1 sweep goes via t1).
         const SellDev<T> &M = fwd ? D.sf : D.sb;
         const int lanes = fwd ? D.wf : D.wb;
-        const T *xin = D.buf;
-        T *bufs[2] = {D.t1, D.t2};
+        const double *xin = D.buf;
+        double *bufs[2] = {D.t1, D.t2};
         for (int k = 0; k < nsw; ++k) {
             const bool last = (k == nsw - 1) && nsw >= 2;
-            T *xout = last ? D.buf : bufs[k & 1];
+            double *xout = last ? D.buf : bufs[k & 1];
             if (lanes > 0)
                 ildl_jacobi_csr<T>(s, lanes, D.nB, ptr, idx, val, D.buf, xin, xout);
             else
@@ -365,7 +365,8 @@ class IldlImpl : public IldlImplBase {
             xin = xout;
         }
         if (nsw == 1)
-            JDHIP_CHECK(hipMemcpyAsync(D.buf, D.t1, sizeof(T) * D.nB, hipMemcpyDeviceToDevice, s));
+            JDHIP_CHECK(
+                hipMemcpyAsync(D.buf, D.t1, sizeof(double) * D.nB, hipMemcpyDeviceToDevice, s));
     }
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
@@ -524,17 +525,17 @@ class IldlImpl : public IldlImplBase {
         const int L = (int)lv_.size();
         if (L == 0) {
             // only a dense level: z = Minv r
-            ildl_gather<TV, T>(s, f_.m, ident_(), ones_(), r, dx_);
+            ildl_gather<TV>(s, f_.m, ident_(), ones_(), r, dx_);
             ildl_dense_gemv<T>(s, f_.m, dmat_, dx_, dy_);
-            ildl_scatter<T, TV>(s, f_.m, f_.m, ident_(), ones_(), dy_, dy_, z);
+            ildl_scatter<TV>(s, f_.m, f_.m, ident_(), ones_(), dy_, dy_, z);
             return;
         }
         for (int l = 0; l < L; ++l) {
             DevLevel<T> &D = lv_[l];
-            if (l == 0) ildl_gather<TV, T>(s, D.n, D.perm, D.scal, r, D.buf);
+            if (l == 0) ildl_gather<TV>(s, D.n, D.perm, D.scal, r, D.buf);
             else {
                 const DevLevel<T> &P = lv_[l - 1];
-                ildl_gather<T, T>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
+                ildl_gather<double>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
             }
             tri_solve(D, true);
             if (D.nC > 0 && D.wc > 0)
@@ -542,9 +543,9 @@ class IldlImpl : public IldlImplBase {
             else if (D.nC > 0)
                 ildl_sub_sell<T>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
                                  D.sc.col, D.buf, D.buf + D.nB, xcd_);
-            ildl_dscale<T>(s, D.nB, D.dinv, D.buf);
+            ildl_dscale(s, D.nB, D.dinv, D.buf);
         }
-        const T *zc = nullptr;
+        const double *zc = nullptr;
         if (f_.m > 0) {
             const DevLevel<T> &P = lv_[L - 1];
             ildl_dense_gemv<T>(s, f_.m, dmat_, P.buf + P.nB, dy_);
@@ -558,8 +559,8 @@ class IldlImpl : public IldlImplBase {
                 ildl_sub_sell<T>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
                                  D.sct.val, D.sct.col, zc, D.buf, xcd_);
             tri_solve(D, false);
-            if (l == 0) ildl_scatter<T, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
-            else ildl_scatter<T, T>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
+            if (l == 0) ildl_scatter<TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
+            else ildl_scatter<double>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
             zc = D.out;
         }
     }
@@ -593,15 +594,16 @@ class IldlImpl : public IldlImplBase {
         // per level: one pass over L_B per sweep and direction (values,
         // indices, row map, b, x, result), L_CB and L_CB^T once,
         // gather/scatter (perm + scal + in/out), D^-1, dense inverse.
-        const long s = sizeof(T);
+        // Factor values are sizeof(T) bytes, vectors and row scalars 8.
+        const long s = sizeof(T), v = 8;
         long b = 0;
         for (const auto &D : lv_) {
             const long k = D.k > 0 ? D.k : 1;
-            b += 2 * k * ((s + 4) * D.nnz_lb + 4L * D.nB + 3 * s * D.nB);
-            b += 2 * ((s + 4) * D.nnz_lc + 4L * (D.nC + D.nB + 2) + s * (D.nB + D.nC));
-            b += 2 * (4L + s) * D.n + 2 * s * D.n + 2 * s * D.nB;
+            b += 2 * k * ((s + 4) * D.nnz_lb + 4L * D.nB + 3 * v * D.nB);
+            b += 2 * ((s + 4) * D.nnz_lc + 4L * (D.nC + D.nB + 2) + v * (D.nB + D.nC));
+            b += 2 * (4L + v) * D.n + 2 * v * D.n + 2 * v * D.nB;
         }
-        b += s * (long)f_.m * f_.m + 2 * s * f_.m;
+        b += s * (long)f_.m * f_.m + 2 * v * f_.m;
         return b;
     }
     bool single() const override { return sizeof(T) == 4; }
@@ -610,8 +612,8 @@ class IldlImpl : public IldlImplBase {
         if (lv_.empty() || reps <= 0) return -1;
         const DevLevel<T> &D = lv_[0];
         hipStream_t s = ctx_->stream;
-        JDHIP_CHECK(hipMemsetAsync(D.buf, 0, sizeof(T) * D.n, s));
-        JDHIP_CHECK(hipMemsetAsync(D.t1, 0, sizeof(T) * std::max(D.nB, 1), s));
+        JDHIP_CHECK(hipMemsetAsync(D.buf, 0, sizeof(double) * D.n, s));
+        JDHIP_CHECK(hipMemsetAsync(D.t1, 0, sizeof(double) * std::max(D.nB, 1), s));
         hipEvent_t e0, e1;
         JDHIP_CHECK(hipEventCreate(&e0));
         JDHIP_CHECK(hipEventCreate(&e1));
@@ -631,7 +633,7 @@ class IldlImpl : public IldlImplBase {
         hipEventDestroy(e1);
         *avg_ms = ms / reps;
         const long sz = sizeof(T);
-        *alg_bytes = (sz + 4) * D.nnz_lb + 4L * D.nB + 3 * sz * D.nB;
+        *alg_bytes = (sz + 4) * D.nnz_lb + 4L * D.nB + 3 * 8L * D.nB;
         return 0;
     }
 
@@ -646,12 +648,12 @@ class IldlImpl : public IldlImplBase {
             std::vector<int> id(f_.m);
             for (int i = 0; i < f_.m; ++i) id[i] = i;
             idp_ = track(dev_upload(id));
-            std::vector<T> one(f_.m, T(1));
+            std::vector<double> one(f_.m, 1.0);
             onep_ = track(dev_upload(one));
         }
         return idp_;
     }
-    const T *ones_() {
+    const double *ones_() {
         ident_();
         return onep_;
     }
@@ -671,9 +673,10 @@ class IldlImpl : public IldlImplBase {
     int sigma_;    // SELL-C-sigma sorting window
     std::vector<DevLevel<T>> lv_;
     std::vector<void *> allocs_;
-    T *dmat_ = nullptr, *dx_ = nullptr, *dy_ = nullptr;
+    T *dmat_ = nullptr;
+    double *dx_ = nullptr, *dy_ = nullptr;
     int *idp_ = nullptr;
-    T *onep_ = nullptr;
+    double *onep_ = nullptr;
 };
 
 }  // namespace

@@ -4,10 +4,12 @@
 // on CSR with lane groups of 4/16/64 and a butterfly reduction; the exact
 // level-scheduled solve uses 4-lane groups per row of a level.  Every loop
 // is grid-stride; vector entries are gathered through the column index.
-// Sums are accumulated in double also for the FP32 factor (stored and
-// streamed in float): the substitutions cancel large terms, and float
-// accumulation measurably slows the outer iteration on indefinite
-// matrices, while the extra arithmetic is free in these HBM-bound loops.
+// The factor's values are stored and streamed as T (double, or float on
+// the mixed path); the vectors of the apply and every sum are double: the
+// substitutions cancel large terms, and float intermediates measurably slow
+// the outer iteration on indefinite matrices, while the vector traffic is
+// small next to the factor's and the extra arithmetic is free in these
+// HBM-bound loops.
 #include "ildl_kernels.hpp"
 
 #include <algorithm>
@@ -34,24 +36,26 @@ static inline int blocks_for(long n, int per_block, int cap = 8192) {
     return (int)std::max<long>(1, std::min<long>(b, cap));
 }
 
-template <typename TI, typename T>
+template <typename TI>
 __global__ __launch_bounds__(kB) void k_gather(int n, const int *__restrict__ perm,
-                                               const T *__restrict__ scal,
-                                               const TI *__restrict__ in, T *__restrict__ buf) {
+                                               const double *__restrict__ scal,
+                                               const TI *__restrict__ in,
+                                               double *__restrict__ buf) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
         int p = perm[i];
-        buf[i] = scal[p] * (T)in[p];
+        buf[i] = scal[p] * (double)in[p];
     }
 }
 
-template <typename T, typename TO>
+template <typename TO>
 __global__ __launch_bounds__(kB) void k_scatter(int n, int nB, const int *__restrict__ perm,
-                                                const T *__restrict__ scal,
-                                                const T *__restrict__ buf,
-                                                const T *__restrict__ zc, TO *__restrict__ out) {
+                                                const double *__restrict__ scal,
+                                                const double *__restrict__ buf,
+                                                const double *__restrict__ zc,
+                                                TO *__restrict__ out) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
         int p = perm[i];
-        T v = (i < nB) ? buf[i] : zc[i - nB];
+        double v = (i < nB) ? buf[i] : zc[i - nB];
         out[p] = (TO)(scal[p] * v);
     }
 }
@@ -60,7 +64,7 @@ template <typename T>
 __global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows, int count,
                                                    const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
-                                                   const T *__restrict__ val, T *x) {
+                                                   const T *__restrict__ val, double *x) {
     const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
     for (long t = (long)blockIdx.x * (kB / kLpr) + g; t < count;
          t += (long)gridDim.x * (kB / kLpr)) {
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows,
         for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += (double)val[k] * (double)x[idx[k]];
 #pragma unroll
         for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
-        if (l == 0) x[r] = (T)((double)x[r] - s);
+        if (l == 0) x[r] -= s;
     }
 }
 
@@ -87,8 +91,8 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
                                                     const int *__restrict__ rowid,
                                                     const T *__restrict__ val,
                                                     const int *__restrict__ col,
-                                                    const T *b, const T *xin, T *xout,
-                                                    int xcd) {
+                                                    const double *b, const double *xin,
+                                                    double *xout, int xcd) {
     // b and xout may be the same array (the last sweep writes in place: each
     // lane reads b only at its own row before writing that row)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -102,11 +106,11 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
         double s0 = 0.0, s1 = 0.0;
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
-            s1 += (double)ld<NT>(v + (k + 1) * 64) * (double)xin[ld<NT>(c + (k + 1) * 64)];
+            s0 += (double)ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
+            s1 += (double)ld<NT>(v + (k + 1) * 64) * xin[ld<NT>(c + (k + 1) * 64)];
         }
-        if (k < w) s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
-        xout[r] = (T)((double)b[r] - (s0 + s1));
+        if (k < w) s0 += (double)ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
+        xout[r] = b[r] - (s0 + s1);
     }
 }
 
@@ -116,8 +120,8 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
                                                  const int *__restrict__ rowid,
                                                  const T *__restrict__ val,
                                                  const int *__restrict__ col,
-                                                 const T *__restrict__ x, T *__restrict__ y,
-                                                 int xcd) {
+                                                 const double *__restrict__ x,
+                                                 double *__restrict__ y, int xcd) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < nslices;
          sl += (long)gridDim.x * (kB / 64)) {
@@ -129,11 +133,11 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
         double s0 = 0.0, s1 = 0.0;
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += (double)v[k * 64] * (double)x[c[k * 64]];
-            s1 += (double)v[(k + 1) * 64] * (double)x[c[(k + 1) * 64]];
+            s0 += (double)v[k * 64] * x[c[k * 64]];
+            s1 += (double)v[(k + 1) * 64] * x[c[(k + 1) * 64]];
         }
-        if (k < w) s0 += (double)v[k * 64] * (double)x[c[k * 64]];
-        y[r] = (T)((double)y[r] - (s0 + s1));
+        if (k < w) s0 += (double)v[k * 64] * x[c[k * 64]];
+        y[r] -= s0 + s1;
     }
 }
 
@@ -143,15 +147,15 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
 template <typename T, int LPR>
 __global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
-                                                   const T *__restrict__ val, const T *b,
-                                                   const T *xin, T *xout) {
+                                                   const T *__restrict__ val, const double *b,
+                                                   const double *xin, double *xout) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)xin[idx[k]];
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * xin[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
-        if (l == 0) xout[r] = (T)((double)b[r] - s);
+        if (l == 0) xout[r] = b[r] - s;
     }
 }
 
@@ -159,19 +163,20 @@ template <typename T, int LPR>
 __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict__ ptr,
                                                 const int *__restrict__ idx,
                                                 const T *__restrict__ val,
-                                                const T *__restrict__ x, T *__restrict__ y) {
+                                                const double *__restrict__ x,
+                                                double *__restrict__ y) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[idx[k]];
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * x[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
-        if (l == 0) y[r] = (T)((double)y[r] - s);
+        if (l == 0) y[r] -= s;
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kB) void k_dscale(int n, const T *__restrict__ dinv, T *x) {
+__global__ __launch_bounds__(kB) void k_dscale(int n, const double *__restrict__ dinv,
+                                               double *x) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
         x[i] *= dinv[i];
 }
@@ -179,39 +184,41 @@ __global__ __launch_bounds__(kB) void k_dscale(int n, const T *__restrict__ dinv
 // one wave per row of the dense inverse
 template <typename T>
 __global__ __launch_bounds__(kB) void k_dense_gemv(int m, const T *__restrict__ Minv,
-                                                   const T *__restrict__ x, T *__restrict__ y) {
+                                                   const double *__restrict__ x,
+                                                   double *__restrict__ y) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (long r = (long)blockIdx.x * (kB / 64) + w; r < m; r += (long)gridDim.x * (kB / 64)) {
         const T *row = Minv + r * (long)m;
         double s = 0.0;
-        for (int j = lane; j < m; j += 64) s += (double)row[j] * (double)x[j];
+        for (int j = lane; j < m; j += 64) s += (double)row[j] * x[j];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) y[r] = (T)s;
+        if (lane == 0) y[r] = s;
     }
 }
 
-template <typename TI, typename T>
-void ildl_gather(hipStream_t s, int n, const int *perm, const T *scal, const TI *in, T *buf) {
+template <typename TI>
+void ildl_gather(hipStream_t s, int n, const int *perm, const double *scal, const TI *in,
+                 double *buf) {
     if (n <= 0) return;
-    k_gather<TI, T><<<blocks_for(n, kB), kB, 0, s>>>(n, perm, scal, in, buf);
+    k_gather<TI><<<blocks_for(n, kB), kB, 0, s>>>(n, perm, scal, in, buf);
 }
-template <typename T, typename TO>
-void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const T *scal, const T *buf,
-                  const T *zc, TO *out) {
+template <typename TO>
+void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const double *scal,
+                  const double *buf, const double *zc, TO *out) {
     if (n <= 0) return;
-    k_scatter<T, TO><<<blocks_for(n, kB), kB, 0, s>>>(n, nB, perm, scal, buf, zc, out);
+    k_scatter<TO><<<blocks_for(n, kB), kB, 0, s>>>(n, nB, perm, scal, buf, zc, out);
 }
 template <typename T>
 void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
-                     const T *val, T *x) {
+                     const T *val, double *x) {
     if (count <= 0) return;
     k_trsv_level<T><<<blocks_for(count, kB / kLpr), kB, 0, s>>>(rows, count, ptr, idx, val, x);
 }
 template <typename T>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                            const int *rowid, const T *val, const int *col, const T *b,
-                            const T *xin, T *xout, int xcd) {
+                            const int *rowid, const T *val, const int *col, const double *b,
+                            const double *xin, double *xout, int xcd) {
     if (nslices <= 0) return;
     if (xcd & 2)
         k_jacobi_sell<T, true><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
@@ -222,7 +229,7 @@ void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const 
 }
 template <typename T>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const T *x, T *y,
+                   const int *rowid, const T *val, const int *col, const double *x, double *y,
                    int xcd) {
     if (nslices <= 0) return;
     k_sub_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(nslices, vptr, wid, rowid,
@@ -230,7 +237,7 @@ void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
 }
 template <typename T>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                     const T *val, const T *b, const T *xin, T *xout) {
+                     const T *val, const double *b, const double *xin, double *xout) {
     if (nrows <= 0) return;
     if (lpr >= 64)
         k_jacobi_csr<T, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
@@ -244,7 +251,7 @@ void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const in
 }
 template <typename T>
 void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                  const T *val, const T *x, T *y) {
+                  const T *val, const double *x, double *y) {
     if (nrows <= 0) return;
     if (lpr >= 64)
         k_sub_csr<T, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
@@ -256,48 +263,38 @@ void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *
         k_sub_csr<T, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val, x,
                                                                          y);
 }
-template <typename T>
-void ildl_dscale(hipStream_t s, int n, const T *dinv, T *x) {
+void ildl_dscale(hipStream_t s, int n, const double *dinv, double *x) {
     if (n <= 0) return;
-    k_dscale<T><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
+    k_dscale<<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
 }
 template <typename T>
-void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const T *x, T *y) {
+void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const double *x, double *y) {
     if (m <= 0) return;
     k_dense_gemv<T><<<blocks_for(m, kB / 64, 4096), kB, 0, s>>>(m, Minv, x, y);
 }
 
 #define INST(T)                                                                              \
     template void ildl_trsv_level<T>(hipStream_t, const int *, int, const int *, const int *, \
-                                     const T *, T *);                                        \
+                                     const T *, double *);                                   \
     template void ildl_jacobi_sweep_sell<T>(hipStream_t, int, const long *, const int *,      \
-                                            const int *, const T *, const int *, const T *,  \
-                                            const T *, T *, int);                            \
+                                            const int *, const T *, const int *,             \
+                                            const double *, const double *, double *, int);  \
     template void ildl_sub_sell<T>(hipStream_t, int, const long *, const int *, const int *,  \
-                                   const T *, const int *, const T *, T *, int);             \
+                                   const T *, const int *, const double *, double *, int);   \
     template void ildl_jacobi_csr<T>(hipStream_t, int, int, const int *, const int *,         \
-                                     const T *, const T *, const T *, T *);                  \
+                                     const T *, const double *, const double *, double *);   \
     template void ildl_sub_csr<T>(hipStream_t, int, int, const int *, const int *, const T *, \
-                                  const T *, T *);                                           \
-    template void ildl_dscale<T>(hipStream_t, int, const T *, T *);                          \
-    template void ildl_dense_gemv<T>(hipStream_t, int, const T *, const T *, T *);
+                                  const double *, double *);                                 \
+    template void ildl_dense_gemv<T>(hipStream_t, int, const T *, const double *, double *);
 INST(double)
 INST(float)
-template void ildl_gather<double, double>(hipStream_t, int, const int *, const double *,
-                                          const double *, double *);
-template void ildl_gather<double, float>(hipStream_t, int, const int *, const float *,
-                                         const double *, float *);
-template void ildl_gather<float, float>(hipStream_t, int, const int *, const float *,
-                                        const float *, float *);
-template void ildl_gather<float, double>(hipStream_t, int, const int *, const double *,
-                                         const float *, double *);
-template void ildl_scatter<double, float>(hipStream_t, int, int, const int *, const double *,
-                                          const double *, const double *, float *);
-template void ildl_scatter<double, double>(hipStream_t, int, int, const int *, const double *,
-                                           const double *, const double *, double *);
-template void ildl_scatter<float, double>(hipStream_t, int, int, const int *, const float *,
-                                          const float *, const float *, double *);
-template void ildl_scatter<float, float>(hipStream_t, int, int, const int *, const float *,
-                                         const float *, const float *, float *);
+template void ildl_gather<double>(hipStream_t, int, const int *, const double *, const double *,
+                                  double *);
+template void ildl_gather<float>(hipStream_t, int, const int *, const double *, const float *,
+                                 double *);
+template void ildl_scatter<double>(hipStream_t, int, int, const int *, const double *,
+                                   const double *, const double *, double *);
+template void ildl_scatter<float>(hipStream_t, int, int, const int *, const double *,
+                                  const double *, const double *, float *);
 
 }  // namespace jdhip
