@@ -69,6 +69,7 @@ struct Solver {
     // blocking streams synchronise with the null stream)
     bool ildl_graph = false;
     int ildl_threads = 0;  // host threads of the factorisation (0: all cores, <= 16)
+    int ildl_vec32 = -1;   // FP32 factor: work vectors FP32 (1), FP64 (0), by definiteness (-1)
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -514,6 +515,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         ildl->xcd_remap = ildl_xcd;
         ildl->sell_sigma = ildl_sigma;
         ildl->graphs = ildl_graph;
+        ildl->vec32 = ildl_vec32;
         ildl->attach(h, ildl_sweeps);
         ildl_info = ildl->result;
         totaltime =
@@ -958,6 +960,7 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_xcd") S.ildl_xcd = (int)value;
     else if (k == "ildl_sigma") S.ildl_sigma = (int)value;
     else if (k == "ildl_threads") S.ildl_threads = (int)value;
+    else if (k == "ildl_vec32") S.ildl_vec32 = (int)value;
     else if (k == "ildl_graph") {
         S.ildl_graph = value != 0.0;
         if (S.ildl && S.ildl->ready()) S.ildl->impl()->set_graphs(S.ildl_graph);
@@ -1119,6 +1122,7 @@ int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
         S.ildl->xcd_remap = S.ildl_xcd;
         S.ildl->sell_sigma = S.ildl_sigma;
         S.ildl->graphs = S.ildl_graph;
+        S.ildl->vec32 = S.ildl_vec32;
         S.ildl->attach(f->h, sweeps);
         S.factdgl = -1;
         S.n = f->h.single ? (f->h.ff.levels.empty() ? f->h.ff.m : f->h.ff.levels[0].n)

@@ -158,7 +158,7 @@ inline int pick_lanes(long nnz, int rows) {
     return 0;
 }
 
-template <typename T>
+template <typename T, typename W>
 struct DevLevel {
     int n = 0, nB = 0, nC = 0;
     int *perm = nullptr;
@@ -173,7 +173,7 @@ struct DevLevel {
     T *lct_val = nullptr;
     std::vector<int> fwd_ptr, bwd_ptr;  // host level pointers
     int *fwd_rows = nullptr, *bwd_rows = nullptr;
-    double *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;  // work vectors
+    W *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;  // work vectors
     long nnz_lb = 0, nnz_lc = 0;
     SellDev<T> sf, sb;  // sweep copies of L_B (forward) and L_B^T (backward)
     int k = 0;          // this level's triangular solves: 0 exact, k Jacobi sweeps
@@ -183,7 +183,8 @@ struct DevLevel {
     SellDev<T> sc, sct; // L_CB and L_CB^T
 };
 
-template <typename T>
+// T: type of the stored factor values, W: type of the apply's work vectors
+template <typename T, typename W>
 class IldlImpl : public IldlImplBase {
   public:
     IldlImpl(DevCtx *ctx, IldlMultilevel<T> &&f, int sweeps, int storage_order, int xcd,
@@ -228,7 +229,7 @@ class IldlImpl : public IldlImplBase {
             }
             for (int t = 0; t < n; ++t) qinv[q[t]] = t;
 
-            DevLevel<T> D;
+            DevLevel<T, W> D;
             D.n = n;
             D.nB = nB;
             D.nC = nC;
@@ -297,10 +298,10 @@ class IldlImpl : public IldlImplBase {
             D.lct_idx = track(dev_upload(ti));
             D.lct_val = track(dev_upload(tv));
             D.wct = pick_lanes((long)ti.size(), nB);
-            D.buf = track(dev_alloc<double>(n));
-            D.out = track(dev_alloc<double>(n));
-            D.t1 = track(dev_alloc<double>(std::max(nB, 1)));
-            D.t2 = track(dev_alloc<double>(std::max(nB, 1)));
+            D.buf = track(dev_alloc<W>(n));
+            D.out = track(dev_alloc<W>(n));
+            D.t1 = track(dev_alloc<W>(std::max(nB, 1)));
+            D.t2 = track(dev_alloc<W>(std::max(nB, 1)));
             D.nnz_lb = (long)L.lb_idx.size();
             D.nnz_lc = (long)L.lc_idx.size();
             res.fwd_levels.push_back((int)D.fwd_ptr.size() - 1);
@@ -326,14 +327,14 @@ class IldlImpl : public IldlImplBase {
                     mv[(size_t)a * m + b] = (T)f_.dinv[(size_t)oa * m + ob];
                 }
             dmat_ = track(dev_upload(mv));
-            dx_ = track(dev_alloc<double>(m));
-            dy_ = track(dev_alloc<double>(m));
+            dx_ = track(dev_alloc<W>(m));
+            dy_ = track(dev_alloc<W>(m));
         }
         for (auto &D : lv_) D.k = sweeps_;
         JDHIP_CHECK(hipDeviceSynchronize());
     }
 
-    void tri_solve(const DevLevel<T> &D, bool fwd) {
+    void tri_solve(const DevLevel<T, W> &D, bool fwd) {
         hipStream_t s = ctx_->stream;
         const int *ptr = fwd ? D.lb_ptr : D.lt_ptr;
         const int *idx = fwd ? D.lb_idx : D.lt_idx;
@@ -343,7 +344,7 @@ class IldlImpl : public IldlImplBase {
             const std::vector<int> &lp = fwd ? D.fwd_ptr : D.bwd_ptr;
             const int *rows = fwd ? D.fwd_rows : D.bwd_rows;
             for (size_t l = 1; l + 1 < lp.size(); ++l)  // level 0 rows have no entries
-                ildl_trsv_level<T>(s, rows + lp[l], lp[l + 1] - lp[l], ptr, idx, val, D.buf);
+                ildl_trsv_level<T, W>(s, rows + lp[l], lp[l + 1] - lp[l], ptr, idx, val, D.buf);
             return;
         }
         // Jacobi sweeps x <- b - N x from x = b (b = D.buf).  The first sweep
@@ -352,21 +353,21 @@ class IldlImpl : public IldlImplBase {
         // the other buffer (needs at least 2 sweeps; 1 sweep goes via t1).
         const SellDev<T> &M = fwd ? D.sf : D.sb;
         const int lanes = fwd ? D.wf : D.wb;
-        const double *xin = D.buf;
-        double *bufs[2] = {D.t1, D.t2};
+        const W *xin = D.buf;
+        W *bufs[2] = {D.t1, D.t2};
         for (int k = 0; k < nsw; ++k) {
             const bool last = (k == nsw - 1) && nsw >= 2;
-            double *xout = last ? D.buf : bufs[k & 1];
+            W *xout = last ? D.buf : bufs[k & 1];
             if (lanes > 0)
-                ildl_jacobi_csr<T>(s, lanes, D.nB, ptr, idx, val, D.buf, xin, xout);
+                ildl_jacobi_csr<T, W>(s, lanes, D.nB, ptr, idx, val, D.buf, xin, xout);
             else
-                ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col,
-                                          D.buf, xin, xout, xcd_);
+                ildl_jacobi_sweep_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col,
+                                             D.buf, xin, xout, xcd_);
             xin = xout;
         }
         if (nsw == 1)
             JDHIP_CHECK(
-                hipMemcpyAsync(D.buf, D.t1, sizeof(double) * D.nB, hipMemcpyDeviceToDevice, s));
+                hipMemcpyAsync(D.buf, D.t1, sizeof(W) * D.nB, hipMemcpyDeviceToDevice, s));
     }
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
@@ -525,42 +526,43 @@ class IldlImpl : public IldlImplBase {
         const int L = (int)lv_.size();
         if (L == 0) {
             // only a dense level: z = Minv r
-            ildl_gather<TV>(s, f_.m, ident_(), ones_(), r, dx_);
-            ildl_dense_gemv<T>(s, f_.m, dmat_, dx_, dy_);
-            ildl_scatter<TV>(s, f_.m, f_.m, ident_(), ones_(), dy_, dy_, z);
+            ildl_gather<TV, W>(s, f_.m, ident_(), ones_(), r, dx_);
+            ildl_dense_gemv<T, W>(s, f_.m, dmat_, dx_, dy_);
+            ildl_scatter<W, TV>(s, f_.m, f_.m, ident_(), ones_(), dy_, dy_, z);
             return;
         }
         for (int l = 0; l < L; ++l) {
-            DevLevel<T> &D = lv_[l];
-            if (l == 0) ildl_gather<TV>(s, D.n, D.perm, D.scal, r, D.buf);
+            DevLevel<T, W> &D = lv_[l];
+            if (l == 0) ildl_gather<TV, W>(s, D.n, D.perm, D.scal, r, D.buf);
             else {
-                const DevLevel<T> &P = lv_[l - 1];
-                ildl_gather<double>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
+                const DevLevel<T, W> &P = lv_[l - 1];
+                ildl_gather<W, W>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
             }
             tri_solve(D, true);
             if (D.nC > 0 && D.wc > 0)
-                ildl_sub_csr<T>(s, D.wc, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, D.buf, D.buf + D.nB);
+                ildl_sub_csr<T, W>(s, D.wc, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, D.buf,
+                                   D.buf + D.nB);
             else if (D.nC > 0)
-                ildl_sub_sell<T>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
+                ildl_sub_sell<T, W>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
                                  D.sc.col, D.buf, D.buf + D.nB, xcd_);
-            ildl_dscale(s, D.nB, D.dinv, D.buf);
+            ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
         }
-        const double *zc = nullptr;
+        const W *zc = nullptr;
         if (f_.m > 0) {
-            const DevLevel<T> &P = lv_[L - 1];
-            ildl_dense_gemv<T>(s, f_.m, dmat_, P.buf + P.nB, dy_);
+            const DevLevel<T, W> &P = lv_[L - 1];
+            ildl_dense_gemv<T, W>(s, f_.m, dmat_, P.buf + P.nB, dy_);
             zc = dy_;
         }
         for (int l = L - 1; l >= 0; --l) {
-            DevLevel<T> &D = lv_[l];
+            DevLevel<T, W> &D = lv_[l];
             if (D.nC > 0 && D.wct > 0)
-                ildl_sub_csr<T>(s, D.wct, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, zc, D.buf);
+                ildl_sub_csr<T, W>(s, D.wct, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, zc, D.buf);
             else if (D.nC > 0)
-                ildl_sub_sell<T>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
+                ildl_sub_sell<T, W>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
                                  D.sct.val, D.sct.col, zc, D.buf, xcd_);
             tri_solve(D, false);
-            if (l == 0) ildl_scatter<TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
-            else ildl_scatter<double>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
+            if (l == 0) ildl_scatter<W, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
+            else ildl_scatter<W, W>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
             zc = D.out;
         }
     }
@@ -594,8 +596,8 @@ class IldlImpl : public IldlImplBase {
         // per level: one pass over L_B per sweep and direction (values,
         // indices, row map, b, x, result), L_CB and L_CB^T once,
         // gather/scatter (perm + scal + in/out), D^-1, dense inverse.
-        // Factor values are sizeof(T) bytes, vectors and row scalars 8.
-        const long s = sizeof(T), v = 8;
+        // Factor values are sizeof(T) bytes, vectors sizeof(W).
+        const long s = sizeof(T), v = sizeof(W);
         long b = 0;
         for (const auto &D : lv_) {
             const long k = D.k > 0 ? D.k : 1;
@@ -610,17 +612,17 @@ class IldlImpl : public IldlImplBase {
 
     int time_sweep(int reps, double *avg_ms, long *alg_bytes) override {
         if (lv_.empty() || reps <= 0) return -1;
-        const DevLevel<T> &D = lv_[0];
+        const DevLevel<T, W> &D = lv_[0];
         hipStream_t s = ctx_->stream;
-        JDHIP_CHECK(hipMemsetAsync(D.buf, 0, sizeof(double) * D.n, s));
-        JDHIP_CHECK(hipMemsetAsync(D.t1, 0, sizeof(double) * std::max(D.nB, 1), s));
+        JDHIP_CHECK(hipMemsetAsync(D.buf, 0, sizeof(W) * D.n, s));
+        JDHIP_CHECK(hipMemsetAsync(D.t1, 0, sizeof(W) * std::max(D.nB, 1), s));
         hipEvent_t e0, e1;
         JDHIP_CHECK(hipEventCreate(&e0));
         JDHIP_CHECK(hipEventCreate(&e1));
         const SellDev<T> &M = D.sf;
         auto one = [&]() {
-            ildl_jacobi_sweep_sell<T>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, D.buf,
-                                      D.t1, D.t2, xcd_);
+            ildl_jacobi_sweep_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col,
+                                         D.buf, D.t1, D.t2, xcd_);
         };
         one();
         JDHIP_CHECK(hipEventRecord(e0, s));
@@ -633,7 +635,7 @@ class IldlImpl : public IldlImplBase {
         hipEventDestroy(e1);
         *avg_ms = ms / reps;
         const long sz = sizeof(T);
-        *alg_bytes = (sz + 4) * D.nnz_lb + 4L * D.nB + 3 * 8L * D.nB;
+        *alg_bytes = (sz + 4) * D.nnz_lb + 4L * D.nB + 3L * (long)sizeof(W) * D.nB;
         return 0;
     }
 
@@ -671,10 +673,10 @@ class IldlImpl : public IldlImplBase {
     int storage_;  // 0: factor order, 1: original-index order per level
     int xcd_;      // XCD-contiguous block order in the SELL kernels
     int sigma_;    // SELL-C-sigma sorting window
-    std::vector<DevLevel<T>> lv_;
+    std::vector<DevLevel<T, W>> lv_;
     std::vector<void *> allocs_;
     T *dmat_ = nullptr;
-    double *dx_ = nullptr, *dy_ = nullptr;
+    W *dx_ = nullptr, *dy_ = nullptr;
     int *idp_ = nullptr;
     double *onep_ = nullptr;
 };
@@ -761,14 +763,29 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     result.fwd_levels.clear();
     result.bwd_levels.clear();
     const int k0 = sweeps < 0 ? 0 : sweeps;
-    if (h.single) {
+    // An FP32 factor streams float values; its work vectors stay FP64 on
+    // indefinite problems (float intermediates cost the Anderson L=100
+    // interior solve more than 4000 MATVECs instead of 2186 in the oracle
+    // with exact solves) and are FP32 on definite ones, where they cost
+    // nothing measurable and save a tenth of the apply's time (128^3
+    // Laplacian).
+    const bool w32 = vec32 > 0 || (vec32 < 0 && h.spd);
+    if (h.single && w32) {
         IldlMultilevel<float> f = h.ff;
-        auto *p = new IldlImpl<float>(ctx_, std::move(f), k0, storage_order, xcd_remap, sell_sigma);
+        auto *p = new IldlImpl<float, float>(ctx_, std::move(f), k0, storage_order, xcd_remap,
+                                             sell_sigma);
+        impl_.reset(p);
+        p->upload(result);
+    } else if (h.single) {
+        IldlMultilevel<float> f = h.ff;
+        auto *p = new IldlImpl<float, double>(ctx_, std::move(f), k0, storage_order, xcd_remap,
+                                              sell_sigma);
         impl_.reset(p);
         p->upload(result);
     } else {
         IldlMultilevel<double> f = h.fd;
-        auto *p = new IldlImpl<double>(ctx_, std::move(f), k0, storage_order, xcd_remap, sell_sigma);
+        auto *p = new IldlImpl<double, double>(ctx_, std::move(f), k0, storage_order, xcd_remap,
+                                               sell_sigma);
         impl_.reset(p);
         p->upload(result);
     }

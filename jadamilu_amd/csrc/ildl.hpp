@@ -93,6 +93,9 @@ class IldlPrecond {
     int xcd_remap = 1;
     int sell_sigma = 256;  // SELL-C-sigma sorting window of the sweep copies
     bool graphs = false;   // replay the apply from captured HIP graphs
+    // work vectors of an FP32 factor's apply: -1 FP32 for a definite
+    // problem and FP64 otherwise, 0 always FP64, 1 always FP32
+    int vec32 = -1;
 
   private:
     DevCtx *ctx_;

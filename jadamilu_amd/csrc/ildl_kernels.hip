@@ -5,11 +5,11 @@
 // level-scheduled solve uses 4-lane groups per row of a level.  Every loop
 // is grid-stride; vector entries are gathered through the column index.
 // The factor's values are stored and streamed as T (double, or float on
-// the mixed path); the vectors of the apply and every sum are double: the
-// substitutions cancel large terms, and float intermediates measurably slow
-// the outer iteration on indefinite matrices, while the vector traffic is
-// small next to the factor's and the extra arithmetic is free in these
-// HBM-bound loops.
+// the mixed path), the apply's work vectors as W; every sum is double.  On
+// indefinite matrices the mixed path keeps W = double: the substitutions
+// cancel large terms, and float intermediates measurably slow the outer
+// iteration there, while the vector traffic is small next to the
+// factor's and the extra arithmetic is free in these HBM-bound loops.
 #include "ildl_kernels.hpp"
 
 #include <algorithm>
@@ -36,35 +36,35 @@ static inline int blocks_for(long n, int per_block, int cap = 8192) {
     return (int)std::max<long>(1, std::min<long>(b, cap));
 }
 
-template <typename TI>
+template <typename TI, typename W>
 __global__ __launch_bounds__(kB) void k_gather(int n, const int *__restrict__ perm,
                                                const double *__restrict__ scal,
                                                const TI *__restrict__ in,
-                                               double *__restrict__ buf) {
+                                               W *__restrict__ buf) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
         int p = perm[i];
-        buf[i] = scal[p] * (double)in[p];
+        buf[i] = (W)(scal[p] * (double)in[p]);
     }
 }
 
-template <typename TO>
+template <typename W, typename TO>
 __global__ __launch_bounds__(kB) void k_scatter(int n, int nB, const int *__restrict__ perm,
                                                 const double *__restrict__ scal,
-                                                const double *__restrict__ buf,
-                                                const double *__restrict__ zc,
+                                                const W *__restrict__ buf,
+                                                const W *__restrict__ zc,
                                                 TO *__restrict__ out) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
         int p = perm[i];
-        double v = (i < nB) ? buf[i] : zc[i - nB];
+        double v = (i < nB) ? (double)buf[i] : (double)zc[i - nB];
         out[p] = (TO)(scal[p] * v);
     }
 }
 
-template <typename T>
+template <typename T, typename W>
 __global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows, int count,
                                                    const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
-                                                   const T *__restrict__ val, double *x) {
+                                                   const T *__restrict__ val, W *x) {
     const int g = threadIdx.x / kLpr, l = threadIdx.x % kLpr;
     for (long t = (long)blockIdx.x * (kB / kLpr) + g; t < count;
          t += (long)gridDim.x * (kB / kLpr)) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kB) void k_trsv_level(const int *__restrict__ rows,
         for (int k = ptr[r] + l; k < ptr[r + 1]; k += kLpr) s += (double)val[k] * (double)x[idx[k]];
 #pragma unroll
         for (int o = kLpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLpr);
-        if (l == 0) x[r] -= s;
+        if (l == 0) x[r] = (W)((double)x[r] - s);
     }
 }
 
@@ -85,14 +85,14 @@ __device__ __forceinline__ X ld(const X *p) {
     else return *p;
 }
 
-template <typename T, bool NT>
+template <typename T, typename W, bool NT>
 __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__restrict__ vptr,
                                                     const int *__restrict__ wid,
                                                     const int *__restrict__ rowid,
                                                     const T *__restrict__ val,
                                                     const int *__restrict__ col,
-                                                    const double *b, const double *xin,
-                                                    double *xout, int xcd) {
+                                                    const W *b, const W *xin, W *xout,
+                                                    int xcd) {
     // b and xout may be the same array (the last sweep writes in place: each
     // lane reads b only at its own row before writing that row)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -106,22 +106,22 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
         double s0 = 0.0, s1 = 0.0;
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += (double)ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
-            s1 += (double)ld<NT>(v + (k + 1) * 64) * xin[ld<NT>(c + (k + 1) * 64)];
+            s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
+            s1 += (double)ld<NT>(v + (k + 1) * 64) * (double)xin[ld<NT>(c + (k + 1) * 64)];
         }
-        if (k < w) s0 += (double)ld<NT>(v + k * 64) * xin[ld<NT>(c + k * 64)];
-        xout[r] = b[r] - (s0 + s1);
+        if (k < w) s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
+        xout[r] = (W)((double)b[r] - (s0 + s1));
     }
 }
 
-template <typename T>
+template <typename T, typename W>
 __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__restrict__ vptr,
                                                  const int *__restrict__ wid,
                                                  const int *__restrict__ rowid,
                                                  const T *__restrict__ val,
                                                  const int *__restrict__ col,
-                                                 const double *__restrict__ x,
-                                                 double *__restrict__ y, int xcd) {
+                                                 const W *__restrict__ x, W *__restrict__ y,
+                                                 int xcd) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < nslices;
          sl += (long)gridDim.x * (kB / 64)) {
@@ -133,168 +133,174 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
         double s0 = 0.0, s1 = 0.0;
         int k = 0;
         for (; k + 1 < w; k += 2) {
-            s0 += (double)v[k * 64] * x[c[k * 64]];
-            s1 += (double)v[(k + 1) * 64] * x[c[(k + 1) * 64]];
+            s0 += (double)v[k * 64] * (double)x[c[k * 64]];
+            s1 += (double)v[(k + 1) * 64] * (double)x[c[(k + 1) * 64]];
         }
-        if (k < w) s0 += (double)v[k * 64] * x[c[k * 64]];
-        y[r] -= s0 + s1;
+        if (k < w) s0 += (double)v[k * 64] * (double)x[c[k * 64]];
+        y[r] = (W)((double)y[r] - (s0 + s1));
     }
 }
 
 // CSR forms for matrices with long rows (LPR lanes per row, xor-butterfly
 // reduction inside the lane group): with one lane per row such levels keep
 // too few waves in flight and each lane walks its row serially.
-template <typename T, int LPR>
+template <typename T, typename W, int LPR>
 __global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
-                                                   const T *__restrict__ val, const double *b,
-                                                   const double *xin, double *xout) {
+                                                   const T *__restrict__ val, const W *b,
+                                                   const W *xin, W *xout) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * xin[idx[k]];
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR)
+            s += (double)val[k] * (double)xin[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
-        if (l == 0) xout[r] = b[r] - s;
+        if (l == 0) xout[r] = (W)((double)b[r] - s);
     }
 }
 
-template <typename T, int LPR>
+template <typename T, typename W, int LPR>
 __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict__ ptr,
                                                 const int *__restrict__ idx,
                                                 const T *__restrict__ val,
-                                                const double *__restrict__ x,
-                                                double *__restrict__ y) {
+                                                const W *__restrict__ x, W *__restrict__ y) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * x[idx[k]];
+        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
-        if (l == 0) y[r] -= s;
+        if (l == 0) y[r] = (W)((double)y[r] - s);
     }
 }
 
-__global__ __launch_bounds__(kB) void k_dscale(int n, const double *__restrict__ dinv,
-                                               double *x) {
+template <typename W>
+__global__ __launch_bounds__(kB) void k_dscale(int n, const double *__restrict__ dinv, W *x) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
-        x[i] *= dinv[i];
+        x[i] = (W)((double)x[i] * dinv[i]);
 }
 
 // one wave per row of the dense inverse
-template <typename T>
+template <typename T, typename W>
 __global__ __launch_bounds__(kB) void k_dense_gemv(int m, const T *__restrict__ Minv,
-                                                   const double *__restrict__ x,
-                                                   double *__restrict__ y) {
+                                                   const W *__restrict__ x, W *__restrict__ y) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (long r = (long)blockIdx.x * (kB / 64) + w; r < m; r += (long)gridDim.x * (kB / 64)) {
         const T *row = Minv + r * (long)m;
         double s = 0.0;
-        for (int j = lane; j < m; j += 64) s += (double)row[j] * x[j];
+        for (int j = lane; j < m; j += 64) s += (double)row[j] * (double)x[j];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) y[r] = s;
+        if (lane == 0) y[r] = (W)s;
     }
 }
 
-template <typename TI>
+template <typename TI, typename W>
 void ildl_gather(hipStream_t s, int n, const int *perm, const double *scal, const TI *in,
-                 double *buf) {
+                 W *buf) {
     if (n <= 0) return;
-    k_gather<TI><<<blocks_for(n, kB), kB, 0, s>>>(n, perm, scal, in, buf);
+    k_gather<TI, W><<<blocks_for(n, kB), kB, 0, s>>>(n, perm, scal, in, buf);
 }
-template <typename TO>
+template <typename W, typename TO>
 void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const double *scal,
-                  const double *buf, const double *zc, TO *out) {
+                  const W *buf, const W *zc, TO *out) {
     if (n <= 0) return;
-    k_scatter<TO><<<blocks_for(n, kB), kB, 0, s>>>(n, nB, perm, scal, buf, zc, out);
+    k_scatter<W, TO><<<blocks_for(n, kB), kB, 0, s>>>(n, nB, perm, scal, buf, zc, out);
 }
-template <typename T>
+template <typename T, typename W>
 void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
-                     const T *val, double *x) {
+                     const T *val, W *x) {
     if (count <= 0) return;
-    k_trsv_level<T><<<blocks_for(count, kB / kLpr), kB, 0, s>>>(rows, count, ptr, idx, val, x);
+    k_trsv_level<T, W><<<blocks_for(count, kB / kLpr), kB, 0, s>>>(rows, count, ptr, idx, val,
+                                                                   x);
 }
-template <typename T>
+template <typename T, typename W>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                            const int *rowid, const T *val, const int *col, const double *b,
-                            const double *xin, double *xout, int xcd) {
+                            const int *rowid, const T *val, const int *col, const W *b,
+                            const W *xin, W *xout, int xcd) {
     if (nslices <= 0) return;
     if (xcd & 2)
-        k_jacobi_sell<T, true><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
+        k_jacobi_sell<T, W, true><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
             nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd & 1);
     else
-        k_jacobi_sell<T, false><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
+        k_jacobi_sell<T, W, false><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
             nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd & 1);
 }
-template <typename T>
+template <typename T, typename W>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const double *x, double *y,
-                   int xcd) {
+                   const int *rowid, const T *val, const int *col, const W *x, W *y, int xcd) {
     if (nslices <= 0) return;
-    k_sub_sell<T><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(nslices, vptr, wid, rowid,
-                                                                      val, col, x, y, xcd);
+    k_sub_sell<T, W><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(nslices, vptr, wid,
+                                                                         rowid, val, col, x, y,
+                                                                         xcd);
 }
-template <typename T>
+template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                     const T *val, const double *b, const double *xin, double *xout) {
+                     const T *val, const W *b, const W *xin, W *xout) {
     if (nrows <= 0) return;
     if (lpr >= 64)
-        k_jacobi_csr<T, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
-                                                                              val, b, xin, xout);
+        k_jacobi_csr<T, W, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(
+            nrows, ptr, idx, val, b, xin, xout);
     else if (lpr >= 16)
-        k_jacobi_csr<T, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
-                                                                              val, b, xin, xout);
+        k_jacobi_csr<T, W, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(
+            nrows, ptr, idx, val, b, xin, xout);
     else
-        k_jacobi_csr<T, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
-                                                                            b, xin, xout);
+        k_jacobi_csr<T, W, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(
+            nrows, ptr, idx, val, b, xin, xout);
 }
-template <typename T>
+template <typename T, typename W>
 void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                  const T *val, const double *x, double *y) {
+                  const T *val, const W *x, W *y) {
     if (nrows <= 0) return;
     if (lpr >= 64)
-        k_sub_csr<T, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
-                                                                           x, y);
+        k_sub_csr<T, W, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
+                                                                              val, x, y);
     else if (lpr >= 16)
-        k_sub_csr<T, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val,
-                                                                           x, y);
+        k_sub_csr<T, W, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
+                                                                              val, x, y);
     else
-        k_sub_csr<T, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val, x,
-                                                                         y);
+        k_sub_csr<T, W, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
+                                                                            val, x, y);
 }
-void ildl_dscale(hipStream_t s, int n, const double *dinv, double *x) {
+template <typename W>
+void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x) {
     if (n <= 0) return;
-    k_dscale<<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
+    k_dscale<W><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
 }
-template <typename T>
-void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const double *x, double *y) {
+template <typename T, typename W>
+void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y) {
     if (m <= 0) return;
-    k_dense_gemv<T><<<blocks_for(m, kB / 64, 4096), kB, 0, s>>>(m, Minv, x, y);
+    k_dense_gemv<T, W><<<blocks_for(m, kB / 64, 4096), kB, 0, s>>>(m, Minv, x, y);
 }
 
-#define INST(T)                                                                              \
-    template void ildl_trsv_level<T>(hipStream_t, const int *, int, const int *, const int *, \
-                                     const T *, double *);                                   \
-    template void ildl_jacobi_sweep_sell<T>(hipStream_t, int, const long *, const int *,      \
-                                            const int *, const T *, const int *,             \
-                                            const double *, const double *, double *, int);  \
-    template void ildl_sub_sell<T>(hipStream_t, int, const long *, const int *, const int *,  \
-                                   const T *, const int *, const double *, double *, int);   \
-    template void ildl_jacobi_csr<T>(hipStream_t, int, int, const int *, const int *,         \
-                                     const T *, const double *, const double *, double *);   \
-    template void ildl_sub_csr<T>(hipStream_t, int, int, const int *, const int *, const T *, \
-                                  const double *, double *);                                 \
-    template void ildl_dense_gemv<T>(hipStream_t, int, const T *, const double *, double *);
-INST(double)
-INST(float)
-template void ildl_gather<double>(hipStream_t, int, const int *, const double *, const double *,
-                                  double *);
-template void ildl_gather<float>(hipStream_t, int, const int *, const double *, const float *,
-                                 double *);
-template void ildl_scatter<double>(hipStream_t, int, int, const int *, const double *,
-                                   const double *, const double *, double *);
-template void ildl_scatter<float>(hipStream_t, int, int, const int *, const double *,
-                                  const double *, const double *, float *);
+#define INST(T, W)                                                                           \
+    template void ildl_trsv_level<T, W>(hipStream_t, const int *, int, const int *,          \
+                                        const int *, const T *, W *);                        \
+    template void ildl_jacobi_sweep_sell<T, W>(hipStream_t, int, const long *, const int *,   \
+                                               const int *, const T *, const int *,          \
+                                               const W *, const W *, W *, int);              \
+    template void ildl_sub_sell<T, W>(hipStream_t, int, const long *, const int *,           \
+                                      const int *, const T *, const int *, const W *, W *,   \
+                                      int);                                                  \
+    template void ildl_jacobi_csr<T, W>(hipStream_t, int, int, const int *, const int *,      \
+                                        const T *, const W *, const W *, W *);               \
+    template void ildl_sub_csr<T, W>(hipStream_t, int, int, const int *, const int *,         \
+                                     const T *, const W *, W *);                             \
+    template void ildl_dense_gemv<T, W>(hipStream_t, int, const T *, const W *, W *);
+INST(double, double)
+INST(float, double)
+INST(float, float)
+template void ildl_dscale<double>(hipStream_t, int, const double *, double *);
+template void ildl_dscale<float>(hipStream_t, int, const double *, float *);
+#define INST_IO(A, B)                                                                        \
+    template void ildl_gather<A, B>(hipStream_t, int, const int *, const double *, const A *, \
+                                    B *);                                                    \
+    template void ildl_scatter<B, A>(hipStream_t, int, int, const int *, const double *,      \
+                                     const B *, const B *, A *);
+INST_IO(double, double)
+INST_IO(double, float)
+INST_IO(float, double)
+INST_IO(float, float)
 
 }  // namespace jdhip

@@ -12,42 +12,43 @@
 
 namespace jdhip {
 
-// Factor values are T (double or float); vectors and sums are double.
-template <typename TI>
+// Factor values are T (double or float), work vectors W (double or float);
+// sums are double.
+template <typename TI, typename W>
 void ildl_gather(hipStream_t s, int n, const int *perm, const double *scal, const TI *in,
-                 double *buf);
-template <typename TO>
+                 W *buf);
+template <typename W, typename TO>
 void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const double *scal,
-                  const double *buf, const double *zc, TO *out);
+                  const W *buf, const W *zc, TO *out);
 // x[rows of one level] -= sum_j M(row, j) x[j]   (unit-diagonal triangular)
-template <typename T>
+template <typename T, typename W>
 void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
-                     const T *val, double *x);
+                     const T *val, W *x);
 // One Jacobi sweep of the unit triangle, xout = b - M xin, on a SELL-C-sigma
 // copy of M: slices of 64 rows (one wave64, one lane per row) taken from
 // windows of rows sorted by length, entries of a slice stored column-major
 // (value and column at vptr[s] + k*64 + lane, k < wid[s]); rowid[s*64 +
 // lane] is the row a lane owns (-1 for padding lanes).  xcd bit 0:
 // XCD-contiguous block order; bit 1: nontemporal loads of the matrix stream.
-template <typename T>
+template <typename T, typename W>
 void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                            const int *rowid, const T *val, const int *col, const double *b,
-                            const double *xin, double *xout, int xcd);
+                            const int *rowid, const T *val, const int *col, const W *b,
+                            const W *xin, W *xout, int xcd);
 // CSR forms with lpr (4, 16 or 64) lanes per row, for matrices with long rows
-template <typename T>
+template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                     const T *val, const double *b, const double *xin, double *xout);
-template <typename T>
+                     const T *val, const W *b, const W *xin, W *xout);
+template <typename T, typename W>
 void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                  const T *val, const double *x, double *y);
+                  const T *val, const W *x, W *y);
 // y[r] -= sum_j M(r, j) x[j] over the rows of a SELL-C-sigma matrix
-template <typename T>
+template <typename T, typename W>
 void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const double *x, double *y,
-                   int xcd);
-void ildl_dscale(hipStream_t s, int n, const double *dinv, double *x);
+                   const int *rowid, const T *val, const int *col, const W *x, W *y, int xcd);
+template <typename W>
+void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x);
 // y = Minv x, Minv m x m row-major
-template <typename T>
-void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const double *x, double *y);
+template <typename T, typename W>
+void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y);
 
 }  // namespace jdhip
