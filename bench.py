@@ -389,6 +389,7 @@ def main():
     pinfo = ctx.ildl_device_info() if args.precond == "ildl" else None
     if pinfo is not None:
         pinfo["level_sweeps"] = ctx.ildl_level_sweeps()
+        pinfo["level_forms"] = ctx.ildl_level_forms()
     k_sw = r.stats.get("precond_sweeps", 0)
     tri_mode = ("exact level-scheduled" if k_sw == 0 else f"up to {k_sw} Jacobi sweeps") + \
         (" (tuned at set-up)" if args.sweeps < 0 else "")

@@ -195,6 +195,13 @@ int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels, lon
  * (0 exact level-scheduled, k Jacobi sweeps) into out[0..cap-1]; returns the
  * number of levels, or -1 without an ILDL preconditioner. */
 int jdhip_ildl_level_sweeps(jdhip_ctx *ctx, int *out, int cap);
+/* Kernel form of each product of each sparse level, four per level (forward
+ * sweep over L_B, backward sweep over L_B^T, L_CB product, L_CB^T product):
+ * 0 one lane per row on a SELL-C-sigma copy, k > 0 CSR with k lanes per row.
+ * Chosen by timing at set-up (option ildl_autotune, default 1), else by the
+ * average row length.  Fills out[0..cap-1]; returns the number of levels, or
+ * -1 without an ILDL preconditioner.  No reference counterpart. */
+int jdhip_ildl_level_forms(jdhip_ctx *ctx, int *out, int cap);
 /* Diagnostic: time reps launches of one Jacobi sweep over the first level's
  * L_B (the largest single kernel of the preconditioner apply) on ctx's stream
  * with HIP events; *alg_bytes = its algorithmic bytes per launch.  No

@@ -89,6 +89,7 @@ SIGNATURES = {
     "jdhip_ildl_device_info": (C.c_int, [_v, _i, _i, _l]),
     "jdhip_ildl_time_sweep": (C.c_int, [_v, C.c_int, _d, _l]),
     "jdhip_ildl_level_sweeps": (C.c_int, [_v, _i, C.c_int]),
+    "jdhip_ildl_level_forms": (C.c_int, [_v, _i, C.c_int]),
     "jdhip_adjoint_csr": (C.c_int, [_v, C.c_int, _v, _v, _v, C.c_long, C.c_int, _d, _v]),
     "jdhip_adjoint_diag": (C.c_int, [_v, C.c_int, _v, C.c_long, C.c_int, _d, _v]),
     "jdhip_last_error": (C.c_char_p, [_v]),

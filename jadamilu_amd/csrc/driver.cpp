@@ -70,6 +70,7 @@ struct Solver {
     bool ildl_graph = false;
     int ildl_threads = 0;  // host threads of the factorisation (0: all cores, <= 16)
     int ildl_vec32 = -1;   // FP32 factor: work vectors FP32 (1), FP64 (0), by definiteness (-1)
+    bool ildl_autotune = true;  // kernel form of each product by timing at set-up
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -516,6 +517,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         ildl->sell_sigma = ildl_sigma;
         ildl->graphs = ildl_graph;
         ildl->vec32 = ildl_vec32;
+        ildl->autotune = ildl_autotune;
         ildl->attach(h, ildl_sweeps);
         ildl_info = ildl->result;
         totaltime =
@@ -961,6 +963,7 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_sigma") S.ildl_sigma = (int)value;
     else if (k == "ildl_threads") S.ildl_threads = (int)value;
     else if (k == "ildl_vec32") S.ildl_vec32 = (int)value;
+    else if (k == "ildl_autotune") S.ildl_autotune = value != 0.0;
     else if (k == "ildl_graph") {
         S.ildl_graph = value != 0.0;
         if (S.ildl && S.ildl->ready()) S.ildl->impl()->set_graphs(S.ildl_graph);
@@ -1001,6 +1004,14 @@ int jdhip_ildl_level_sweeps(jdhip_ctx *ctx, int *out, int cap) {
     std::vector<int> v = S.ildl->impl()->level_sweeps();
     for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
     return (int)v.size();
+}
+
+int jdhip_ildl_level_forms(jdhip_ctx *ctx, int *out, int cap) {
+    Solver &S = *ctx->S;
+    if (!S.ildl || !S.ildl->ready()) return -1;
+    std::vector<int> v = S.ildl->impl()->level_forms();
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size() / 4;
 }
 
 int jdhip_ildl_time_sweep(jdhip_ctx *ctx, int reps, double *avg_ms, long *alg_bytes) {
@@ -1123,6 +1134,7 @@ int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
         S.ildl->sell_sigma = S.ildl_sigma;
         S.ildl->graphs = S.ildl_graph;
         S.ildl->vec32 = S.ildl_vec32;
+        S.ildl->autotune = S.ildl_autotune;
         S.ildl->attach(f->h, sweeps);
         S.factdgl = -1;
         S.n = f->h.single ? (f->h.ff.levels.empty() ? f->h.ff.m : f->h.ff.levels[0].n)

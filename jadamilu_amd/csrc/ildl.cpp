@@ -351,23 +351,111 @@ class IldlImpl : public IldlImplBase {
         // reads b itself; the last writes into D.buf, which is safe because
         // every row reads only its own b entry and all x entries come from
         // the other buffer (needs at least 2 sweeps; 1 sweep goes via t1).
-        const SellDev<T> &M = fwd ? D.sf : D.sb;
         const int lanes = fwd ? D.wf : D.wb;
         const W *xin = D.buf;
         W *bufs[2] = {D.t1, D.t2};
         for (int k = 0; k < nsw; ++k) {
             const bool last = (k == nsw - 1) && nsw >= 2;
             W *xout = last ? D.buf : bufs[k & 1];
-            if (lanes > 0)
-                ildl_jacobi_csr<T, W>(s, lanes, D.nB, ptr, idx, val, D.buf, xin, xout);
-            else
-                ildl_jacobi_sweep_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col,
-                                             D.buf, xin, xout, xcd_);
+            sweep_launch(D, fwd, lanes, D.buf, xin, xout);
             xin = xout;
         }
         if (nsw == 1)
             JDHIP_CHECK(
                 hipMemcpyAsync(D.buf, D.t1, sizeof(W) * D.nB, hipMemcpyDeviceToDevice, s));
+    }
+
+    // One Jacobi sweep xout = b - N xin of the level's L_B (fwd) or L_B^T,
+    // in the given kernel form: 0 one lane per row on the SELL-C-sigma copy,
+    // else CSR with that many lanes per row.
+    void sweep_launch(const DevLevel<T, W> &D, bool fwd, int form, const W *b, const W *xin,
+                      W *xout) {
+        hipStream_t s = ctx_->stream;
+        if (form > 0) {
+            ildl_jacobi_csr<T, W>(s, form, D.nB, fwd ? D.lb_ptr : D.lt_ptr,
+                                  fwd ? D.lb_idx : D.lt_idx, fwd ? D.lb_val : D.lt_val, b, xin,
+                                  xout);
+        } else {
+            const SellDev<T> &M = fwd ? D.sf : D.sb;
+            ildl_jacobi_sweep_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, b,
+                                         xin, xout, xcd_);
+        }
+    }
+    // y -= L_CB x (C rows; trans = false) or y -= L_CB^T x (B rows)
+    void sub_launch(const DevLevel<T, W> &D, bool trans, int form, const W *x, W *y) {
+        hipStream_t s = ctx_->stream;
+        if (form > 0) {
+            if (trans)
+                ildl_sub_csr<T, W>(s, form, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, x, y);
+            else
+                ildl_sub_csr<T, W>(s, form, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, x, y);
+        } else {
+            const SellDev<T> &M = trans ? D.sct : D.sc;
+            ildl_sub_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, x, y, xcd_);
+        }
+    }
+
+    // Kernel form of every product of every level (both sweeps, L_CB and
+    // L_CB^T) by measurement: the fastest of one lane per row on the
+    // SELL-C-sigma copy and CSR with 4, 16 or 64 lanes per row, each timed
+    // over three launches on zeroed vectors.  Replaces the row-length
+    // heuristic the upload starts from.
+    void autotune_forms() override {
+        hipStream_t s = ctx_->stream;
+        hipEvent_t e0, e1;
+        JDHIP_CHECK(hipEventCreate(&e0));
+        JDHIP_CHECK(hipEventCreate(&e1));
+        auto timed = [&](auto &&launch) {
+            launch();
+            JDHIP_CHECK(hipEventRecord(e0, s));
+            for (int i = 0; i < 3; ++i) launch();
+            JDHIP_CHECK(hipEventRecord(e1, s));
+            JDHIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
+        const int forms[4] = {0, 4, 16, 64};
+        auto best = [&](auto &&launch_form) {
+            int bf = 0;
+            float bt = 0.f;
+            for (int f : forms) {
+                const float t = timed([&] { launch_form(f); });
+                if (f == 0 || t < bt) {
+                    bt = t;
+                    bf = f;
+                }
+            }
+            return bf;
+        };
+        for (auto &D : lv_) {
+            JDHIP_CHECK(hipMemsetAsync(D.buf, 0, sizeof(W) * D.n, s));
+            JDHIP_CHECK(hipMemsetAsync(D.out, 0, sizeof(W) * D.n, s));
+            JDHIP_CHECK(hipMemsetAsync(D.t1, 0, sizeof(W) * std::max(D.nB, 1), s));
+            JDHIP_CHECK(hipMemsetAsync(D.t2, 0, sizeof(W) * std::max(D.nB, 1), s));
+            if (D.nB > 0) {
+                D.wf = best([&](int f) { sweep_launch(D, true, f, D.buf, D.t1, D.t2); });
+                D.wb = best([&](int f) { sweep_launch(D, false, f, D.buf, D.t1, D.t2); });
+            }
+            if (D.nC > 0) {
+                D.wc = best([&](int f) { sub_launch(D, false, f, D.buf, D.buf + D.nB); });
+                D.wct = best([&](int f) { sub_launch(D, true, f, D.out, D.buf); });
+            }
+        }
+        JDHIP_CHECK(hipStreamSynchronize(s));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        clear_graphs();
+    }
+    std::vector<int> level_forms() const override {
+        std::vector<int> v;
+        for (const auto &D : lv_) {
+            v.push_back(D.wf);
+            v.push_back(D.wb);
+            v.push_back(D.wc);
+            v.push_back(D.wct);
+        }
+        return v;
     }
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
@@ -539,12 +627,7 @@ class IldlImpl : public IldlImplBase {
                 ildl_gather<W, W>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
             }
             tri_solve(D, true);
-            if (D.nC > 0 && D.wc > 0)
-                ildl_sub_csr<T, W>(s, D.wc, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, D.buf,
-                                   D.buf + D.nB);
-            else if (D.nC > 0)
-                ildl_sub_sell<T, W>(s, D.sc.nslices, D.sc.vptr, D.sc.wid, D.sc.rowid, D.sc.val,
-                                 D.sc.col, D.buf, D.buf + D.nB, xcd_);
+            if (D.nC > 0) sub_launch(D, false, D.wc, D.buf, D.buf + D.nB);
             ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
         }
         const W *zc = nullptr;
@@ -555,11 +638,7 @@ class IldlImpl : public IldlImplBase {
         }
         for (int l = L - 1; l >= 0; --l) {
             DevLevel<T, W> &D = lv_[l];
-            if (D.nC > 0 && D.wct > 0)
-                ildl_sub_csr<T, W>(s, D.wct, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, zc, D.buf);
-            else if (D.nC > 0)
-                ildl_sub_sell<T, W>(s, D.sct.nslices, D.sct.vptr, D.sct.wid, D.sct.rowid,
-                                 D.sct.val, D.sct.col, zc, D.buf, xcd_);
+            if (D.nC > 0) sub_launch(D, true, D.wct, zc, D.buf);
             tri_solve(D, false);
             if (l == 0) ildl_scatter<W, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
             else ildl_scatter<W, W>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
@@ -619,11 +698,7 @@ class IldlImpl : public IldlImplBase {
         hipEvent_t e0, e1;
         JDHIP_CHECK(hipEventCreate(&e0));
         JDHIP_CHECK(hipEventCreate(&e1));
-        const SellDev<T> &M = D.sf;
-        auto one = [&]() {
-            ildl_jacobi_sweep_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col,
-                                         D.buf, D.t1, D.t2, xcd_);
-        };
+        auto one = [&]() { sweep_launch(D, true, D.wf, D.buf, D.t1, D.t2); };
         one();
         JDHIP_CHECK(hipEventRecord(e0, s));
         for (int i = 0; i < reps; ++i) one();
@@ -796,6 +871,7 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     // the preconditioner much closer to exact along the near-null space of
     // A - sigma I (Anderson L=100: per-level counts accurate to 3 % on a
     // random vector stalled the iteration), so there the bound is 1e-4.
+    if (autotune) impl_->autotune_forms();
     if (sweeps < 0) impl_->tune_sweeps(h.spd ? 0.03 : 1e-4);
     result.sweeps = impl_->sweeps();
     impl_->set_graphs(graphs);

@@ -57,6 +57,11 @@ class IldlImplBase {
     virtual int time_sweep(int reps, double *avg_ms, long *alg_bytes) = 0;
     virtual int sweeps() const = 0;
     virtual std::vector<int> level_sweeps() const = 0;
+    // Choose every product's kernel form (SELL or CSR lanes) by timing.
+    virtual void autotune_forms() = 0;
+    // per level: forms of the forward sweep, backward sweep, L_CB, L_CB^T
+    // (0 SELL, k lanes per row CSR)
+    virtual std::vector<int> level_forms() const = 0;
     // replay the apply from captured HIP graphs (default on)
     virtual void set_graphs(bool on) = 0;
 };
@@ -96,6 +101,7 @@ class IldlPrecond {
     // work vectors of an FP32 factor's apply: -1 FP32 for a definite
     // problem and FP64 otherwise, 0 always FP64, 1 always FP32
     int vec32 = -1;
+    bool autotune = true;  // kernel forms by measurement (else by row length)
 
   private:
     DevCtx *ctx_;

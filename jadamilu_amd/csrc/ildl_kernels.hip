@@ -103,14 +103,17 @@ __global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__r
         const int w = wid[sl];
         const T *v = val + vptr[sl] + lane;
         const int *c = col + vptr[sl] + lane;
-        double s0 = 0.0, s1 = 0.0;
+        // four independent chains keep four gathers of x in flight per lane
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         int k = 0;
-        for (; k + 1 < w; k += 2) {
+        for (; k + 3 < w; k += 4) {
             s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
             s1 += (double)ld<NT>(v + (k + 1) * 64) * (double)xin[ld<NT>(c + (k + 1) * 64)];
+            s2 += (double)ld<NT>(v + (k + 2) * 64) * (double)xin[ld<NT>(c + (k + 2) * 64)];
+            s3 += (double)ld<NT>(v + (k + 3) * 64) * (double)xin[ld<NT>(c + (k + 3) * 64)];
         }
-        if (k < w) s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
-        xout[r] = (W)((double)b[r] - (s0 + s1));
+        for (; k < w; ++k) s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
+        xout[r] = (W)((double)b[r] - ((s0 + s1) + (s2 + s3)));
     }
 }
 
@@ -130,14 +133,16 @@ __global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__rest
         const int w = wid[sl];
         const T *v = val + vptr[sl] + lane;
         const int *c = col + vptr[sl] + lane;
-        double s0 = 0.0, s1 = 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         int k = 0;
-        for (; k + 1 < w; k += 2) {
+        for (; k + 3 < w; k += 4) {
             s0 += (double)v[k * 64] * (double)x[c[k * 64]];
             s1 += (double)v[(k + 1) * 64] * (double)x[c[(k + 1) * 64]];
+            s2 += (double)v[(k + 2) * 64] * (double)x[c[(k + 2) * 64]];
+            s3 += (double)v[(k + 3) * 64] * (double)x[c[(k + 3) * 64]];
         }
-        if (k < w) s0 += (double)v[k * 64] * (double)x[c[k * 64]];
-        y[r] = (W)((double)y[r] - (s0 + s1));
+        for (; k < w; ++k) s0 += (double)v[k * 64] * (double)x[c[k * 64]];
+        y[r] = (W)((double)y[r] - ((s0 + s1) + (s2 + s3)));
     }
 }
 
@@ -152,6 +157,7 @@ __global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restr
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
+#pragma unroll 4
         for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR)
             s += (double)val[k] * (double)xin[idx[k]];
 #pragma unroll
@@ -168,6 +174,7 @@ __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
+#pragma unroll 4
         for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[idx[k]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);

@@ -274,6 +274,13 @@ class Context:
         k = self._L.jdhip_ildl_level_sweeps(self.h, buf, 64)
         return list(buf[: max(k, 0)])
 
+    def ildl_level_forms(self):
+        """Per sparse level: kernel forms (forward sweep, backward sweep, L_CB,
+        L_CB^T); 0 SELL one lane per row, k CSR with k lanes per row."""
+        buf = (C.c_int * 256)()
+        k = self._L.jdhip_ildl_level_forms(self.h, buf, 256)
+        return [list(buf[4 * l: 4 * l + 4]) for l in range(max(k, 0))]
+
     def ildl_device_info(self):
         f, b, nb = C.c_int(), C.c_int(), C.c_long()
         if self._L.jdhip_ildl_device_info(self.h, C.byref(f), C.byref(b), C.byref(nb)) != 0:
