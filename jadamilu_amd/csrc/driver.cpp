@@ -71,6 +71,8 @@ struct Solver {
     int ildl_threads = 0;  // host threads of the factorisation (0: all cores, <= 16)
     int ildl_vec32 = -1;   // FP32 factor: work vectors FP32 (1), FP64 (0), by definiteness (-1)
     bool ildl_autotune = true;  // kernel form of each product by timing at set-up
+    double ildl_tune_tol = 0.0;  // sweep tuning tolerance (0: by definiteness)
+    int ildl_form = -1;  // >= 0: one kernel form for every product of the factor
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -518,6 +520,8 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         ildl->graphs = ildl_graph;
         ildl->vec32 = ildl_vec32;
         ildl->autotune = ildl_autotune;
+        ildl->tune_tol = ildl_tune_tol;
+        ildl->force_form = ildl_form;
         ildl->attach(h, ildl_sweeps);
         ildl_info = ildl->result;
         totaltime =
@@ -964,6 +968,12 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_threads") S.ildl_threads = (int)value;
     else if (k == "ildl_vec32") S.ildl_vec32 = (int)value;
     else if (k == "ildl_autotune") S.ildl_autotune = value != 0.0;
+    else if (k == "ildl_tune_tol") S.ildl_tune_tol = value;
+    else if (k == "ildl_form") {
+        const int f = (int)value;
+        if (f != -1 && f != 0 && f != 4 && f != 16 && f != 64) return -3;
+        S.ildl_form = f;
+    }
     else if (k == "ildl_graph") {
         S.ildl_graph = value != 0.0;
         if (S.ildl && S.ildl->ready()) S.ildl->impl()->set_graphs(S.ildl_graph);
@@ -1135,6 +1145,8 @@ int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
         S.ildl->graphs = S.ildl_graph;
         S.ildl->vec32 = S.ildl_vec32;
         S.ildl->autotune = S.ildl_autotune;
+        S.ildl->tune_tol = S.ildl_tune_tol;
+        S.ildl->force_form = S.ildl_form;
         S.ildl->attach(f->h, sweeps);
         S.factdgl = -1;
         S.n = f->h.single ? (f->h.ff.levels.empty() ? f->h.ff.m : f->h.ff.levels[0].n)

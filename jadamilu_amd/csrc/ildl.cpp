@@ -8,6 +8,7 @@
 #include "ildl.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -91,18 +92,24 @@ void schedule(int n, const std::vector<int> &ptr, const std::vector<int> &idx, b
 
 template <typename T>
 struct SellHost {
-    std::vector<long> vptr;
-    std::vector<int> wid, rowid, col;
+    std::vector<long> vptr, iptr;
+    std::vector<int> code, rowid, cbase;
+    std::vector<char> idx;
     std::vector<T> val;
+    long nnz16 = 0, nnz32 = 0;  // stored (non-padding) entries by index width
 };
 
 // skip_empty: rows without entries get no lane (products y -= M x, where such
 // rows are no-ops; a sweep must still copy b into them, so it keeps them).
+// A slice stores its columns as 16-bit offsets from its smallest column when
+// they span less than 65536, else as int32; padding entries point at that
+// smallest column (a valid index) with value 0.
 template <typename T>
 SellHost<T> build_sell(int nrows, const std::vector<int> &ptr, const std::vector<int> &idx,
                        const std::vector<T> &val, int sigma, bool skip_empty = false) {
     SellHost<T> h;
     std::vector<int> order(nrows);
+    std::vector<int> cols;
     sigma = std::max(sigma, 64);
     for (int r0 = 0; r0 < nrows; r0 += sigma) {
         const int r1 = std::min(nrows, r0 + sigma);
@@ -115,14 +122,26 @@ SellHost<T> build_sell(int nrows, const std::vector<int> &ptr, const std::vector
                 return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b];
             });
         for (int s0 = w0; s0 < w1; s0 += 64) {
-            int w = 0;
-            for (int r = s0; r < std::min(w1, s0 + 64); ++r)
-                w = std::max(w, ptr[order[r] + 1] - ptr[order[r]]);
+            int w = 0, cmin = INT_MAX, cmax = INT_MIN;
+            for (int r = s0; r < std::min(w1, s0 + 64); ++r) {
+                const int row = order[r];
+                w = std::max(w, ptr[row + 1] - ptr[row]);
+                for (int k = ptr[row]; k < ptr[row + 1]; ++k) {
+                    cmin = std::min(cmin, idx[k]);
+                    cmax = std::max(cmax, idx[k]);
+                }
+            }
+            if (cmin > cmax) cmin = cmax = 0;  // no entries in the slice
+            const bool narrow = (long)cmax - cmin < 65536;
+            long real = 0;
+            for (int r = s0; r < std::min(w1, s0 + 64); ++r) real += ptr[order[r] + 1] - ptr[order[r]];
+            (narrow ? h.nnz16 : h.nnz32) += real;
             const long base = h.val.size();
             h.vptr.push_back(base);
-            h.wid.push_back(w);
+            h.code.push_back(2 * w + (narrow ? 1 : 0));
+            h.cbase.push_back(narrow ? cmin : 0);
             h.val.resize(base + 64L * w, T(0));
-            h.col.resize(base + 64L * w, 0);
+            cols.assign(64L * w, cmin);
             for (int lane = 0; lane < 64; ++lane) {
                 const int t = s0 + lane;
                 const int row = (t < w1) ? order[t] : -1;
@@ -130,21 +149,34 @@ SellHost<T> build_sell(int nrows, const std::vector<int> &ptr, const std::vector
                 if (row < 0) continue;
                 for (int k = 0; k < ptr[row + 1] - ptr[row]; ++k) {
                     h.val[base + (long)k * 64 + lane] = val[ptr[row] + k];
-                    h.col[base + (long)k * 64 + lane] = idx[ptr[row] + k];
+                    cols[(long)k * 64 + lane] = idx[ptr[row] + k];
                 }
+            }
+            // index bytes of the slice, 4-byte aligned
+            long at = (long)h.idx.size();
+            at = (at + 3) & ~3L;
+            h.iptr.push_back(at);
+            if (narrow) {
+                h.idx.resize(at + 2L * cols.size());
+                auto *o = reinterpret_cast<unsigned short *>(h.idx.data() + at);
+                for (size_t e = 0; e < cols.size(); ++e) o[e] = (unsigned short)(cols[e] - cmin);
+            } else {
+                h.idx.resize(at + 4L * cols.size());
+                std::memcpy(h.idx.data() + at, cols.data(), 4 * cols.size());
             }
         }
     }
     h.vptr.push_back((long)h.val.size());
+    if (h.idx.empty()) h.idx.resize(4);
     return h;
 }
 
 template <typename T>
 struct SellDev {
-    int nslices = 0;
-    long *vptr = nullptr;
-    int *wid = nullptr, *rowid = nullptr, *col = nullptr;
+    SellView v;  // device arrays of the layout (see SellView)
     T *val = nullptr;
+    long idx_bytes = 0;
+    long nnz16 = 0, nnz32 = 0;
 };
 
 // Lanes per row for a product with nnz entries over rows rows: one lane per
@@ -377,8 +409,7 @@ class IldlImpl : public IldlImplBase {
                                   xout);
         } else {
             const SellDev<T> &M = fwd ? D.sf : D.sb;
-            ildl_jacobi_sweep_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, b,
-                                         xin, xout, xcd_);
+            ildl_jacobi_sweep_sell<T, W>(s, M.v, M.val, b, xin, xout, xcd_);
         }
     }
     // y -= L_CB x (C rows; trans = false) or y -= L_CB^T x (B rows)
@@ -391,7 +422,7 @@ class IldlImpl : public IldlImplBase {
                 ildl_sub_csr<T, W>(s, form, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, x, y);
         } else {
             const SellDev<T> &M = trans ? D.sct : D.sc;
-            ildl_sub_sell<T, W>(s, M.nslices, M.vptr, M.wid, M.rowid, M.val, M.col, x, y, xcd_);
+            ildl_sub_sell<T, W>(s, M.v, M.val, x, y, xcd_);
         }
     }
 
@@ -447,6 +478,10 @@ class IldlImpl : public IldlImplBase {
         hipEventDestroy(e1);
         clear_graphs();
     }
+    void set_forms(int form) override {
+        for (auto &D : lv_) D.wf = D.wb = D.wc = D.wct = form;
+        clear_graphs();
+    }
     std::vector<int> level_forms() const override {
         std::vector<int> v;
         for (const auto &D : lv_) {
@@ -460,12 +495,17 @@ class IldlImpl : public IldlImplBase {
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
         SellDev<T> d;
-        d.nslices = (int)h.wid.size();
-        d.vptr = track(dev_upload(h.vptr));
-        d.wid = track(dev_upload(h.wid));
-        d.rowid = track(dev_upload(h.rowid));
-        d.col = track(dev_upload(h.col));
+        d.v.nslices = (int)h.code.size();
+        d.v.vptr = track(dev_upload(h.vptr));
+        d.v.iptr = track(dev_upload(h.iptr));
+        d.v.code = track(dev_upload(h.code));
+        d.v.rowid = track(dev_upload(h.rowid));
+        d.v.cbase = track(dev_upload(h.cbase));
+        d.v.idx = track(dev_upload(h.idx));
         d.val = track(dev_upload(h.val));
+        d.idx_bytes = (long)h.idx.size();
+        d.nnz16 = h.nnz16;
+        d.nnz32 = h.nnz32;
         return d;
     }
 
@@ -671,17 +711,26 @@ class IldlImpl : public IldlImplBase {
     void dense_data(double *minv) const override {
         for (size_t i = 0; i < f_.dinv.size(); ++i) minv[i] = (double)(T)f_.dinv[i];
     }
+    // values and column indices one product streams in the given form
+    static long mat_bytes(const SellDev<T> &M, int form, long nnz) {
+        if (form > 0) return ((long)sizeof(T) + 4) * nnz;
+        return (long)sizeof(T) * nnz + 2 * M.nnz16 + 4 * M.nnz32;
+    }
     long apply_bytes() const override {
         // per level: one pass over L_B per sweep and direction (values,
         // indices, row map, b, x, result), L_CB and L_CB^T once,
         // gather/scatter (perm + scal + in/out), D^-1, dense inverse.
         // Factor values are sizeof(T) bytes, vectors sizeof(W).
+        // Matrix bytes follow each product's chosen form (int32 columns in
+        // CSR, 16- or 32-bit per SELL slice).
         const long s = sizeof(T), v = sizeof(W);
         long b = 0;
         for (const auto &D : lv_) {
             const long k = D.k > 0 ? D.k : 1;
-            b += 2 * k * ((s + 4) * D.nnz_lb + 4L * D.nB + 3 * v * D.nB);
-            b += 2 * ((s + 4) * D.nnz_lc + 4L * (D.nC + D.nB + 2) + v * (D.nB + D.nC));
+            b += k * (mat_bytes(D.sf, D.wf, D.nnz_lb) + mat_bytes(D.sb, D.wb, D.nnz_lb)) +
+                 2 * k * (4L * D.nB + 3 * v * D.nB);
+            b += mat_bytes(D.sc, D.wc, D.nnz_lc) + mat_bytes(D.sct, D.wct, D.nnz_lc) +
+                 2 * (4L * (D.nC + D.nB + 2) + v * (D.nB + D.nC));
             b += 2 * (4L + v) * D.n + 2 * v * D.n + 2 * v * D.nB;
         }
         b += s * (long)f_.m * f_.m + 2 * v * f_.m;
@@ -710,7 +759,8 @@ class IldlImpl : public IldlImplBase {
         hipEventDestroy(e1);
         *avg_ms = ms / reps;
         const long sz = sizeof(T);
-        *alg_bytes = (sz + 4) * D.nnz_lb + 4L * D.nB + 3L * (long)sizeof(W) * D.nB;
+        (void)sz;
+        *alg_bytes = mat_bytes(D.sf, D.wf, D.nnz_lb) + 4L * D.nB + 3L * (long)sizeof(W) * D.nB;
         return 0;
     }
 
@@ -871,8 +921,12 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     // the preconditioner much closer to exact along the near-null space of
     // A - sigma I (Anderson L=100: per-level counts accurate to 3 % on a
     // random vector stalled the iteration), so there the bound is 1e-4.
-    if (autotune) impl_->autotune_forms();
-    if (sweeps < 0) impl_->tune_sweeps(h.spd ? 0.03 : 1e-4);
+    // (1e-3 runs Anderson L=100 with a quarter fewer sweeps in as many
+    // MATVECs, but on a 10^3 Anderson matrix the iteration then returned
+    // the 2nd-6th eigenvalues nearest sigma instead of the 1st-5th.)
+    if (force_form >= 0) impl_->set_forms(force_form);
+    else if (autotune) impl_->autotune_forms();
+    if (sweeps < 0) impl_->tune_sweeps(tune_tol > 0.0 ? tune_tol : (h.spd ? 0.03 : 1e-4));
     result.sweeps = impl_->sweeps();
     impl_->set_graphs(graphs);
 }

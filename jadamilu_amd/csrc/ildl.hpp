@@ -59,6 +59,8 @@ class IldlImplBase {
     virtual std::vector<int> level_sweeps() const = 0;
     // Choose every product's kernel form (SELL or CSR lanes) by timing.
     virtual void autotune_forms() = 0;
+    // Set every product's form (0 SELL, 4/16/64 CSR lanes per row).
+    virtual void set_forms(int form) = 0;
     // per level: forms of the forward sweep, backward sweep, L_CB, L_CB^T
     // (0 SELL, k lanes per row CSR)
     virtual std::vector<int> level_forms() const = 0;
@@ -102,6 +104,8 @@ class IldlPrecond {
     // problem and FP64 otherwise, 0 always FP64, 1 always FP32
     int vec32 = -1;
     bool autotune = true;  // kernel forms by measurement (else by row length)
+    int force_form = -1;   // >= 0: this form for every product (tests, experiments)
+    double tune_tol = 0.0;  // sweep tuning tolerance (0: 3e-2 definite, 1e-4 indefinite)
 
   private:
     DevCtx *ctx_;

@@ -85,64 +85,63 @@ __device__ __forceinline__ X ld(const X *p) {
     else return *p;
 }
 
+// Dot product of one SELL lane's row with x: w entries, column-major with
+// stride 64; columns as int32 (CI = int, base 0) or as 16-bit offsets from
+// the slice's base column (CI = unsigned short).  Four independent chains
+// keep four gathers of x in flight per lane.
+template <bool NT, typename T, typename W, typename CI>
+__device__ __forceinline__ double sell_row(const T *v, const CI *c, int base, int w,
+                                           const W *x) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = 0;
+    for (; k + 3 < w; k += 4) {
+        s0 += (double)ld<NT>(v + k * 64) * (double)x[base + (int)ld<NT>(c + k * 64)];
+        s1 += (double)ld<NT>(v + (k + 1) * 64) * (double)x[base + (int)ld<NT>(c + (k + 1) * 64)];
+        s2 += (double)ld<NT>(v + (k + 2) * 64) * (double)x[base + (int)ld<NT>(c + (k + 2) * 64)];
+        s3 += (double)ld<NT>(v + (k + 3) * 64) * (double)x[base + (int)ld<NT>(c + (k + 3) * 64)];
+    }
+    for (; k < w; ++k) s0 += (double)ld<NT>(v + k * 64) * (double)x[base + (int)ld<NT>(c + k * 64)];
+    return (s0 + s1) + (s2 + s3);
+}
+
+template <bool NT, typename T, typename W>
+__device__ __forceinline__ double sell_slice_row(const SellView &M, const T *__restrict__ val,
+                                                 long sl, int lane, const W *x) {
+    const int code = M.code[sl];
+    const T *v = val + M.vptr[sl] + lane;
+    if (code & 1)
+        return sell_row<NT>(v, (const unsigned short *)(M.idx + M.iptr[sl]) + lane, M.cbase[sl],
+                            code >> 1, x);
+    return sell_row<NT>(v, (const int *)(M.idx + M.iptr[sl]) + lane, 0, code >> 1, x);
+}
+
 template <typename T, typename W, bool NT>
-__global__ __launch_bounds__(kB) void k_jacobi_sell(int nslices, const long *__restrict__ vptr,
-                                                    const int *__restrict__ wid,
-                                                    const int *__restrict__ rowid,
-                                                    const T *__restrict__ val,
-                                                    const int *__restrict__ col,
+__global__ __launch_bounds__(kB) void k_jacobi_sell(SellView M, const T *__restrict__ val,
                                                     const W *b, const W *xin, W *xout,
                                                     int xcd) {
     // b and xout may be the same array (the last sweep writes in place: each
     // lane reads b only at its own row before writing that row)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < nslices;
+    for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < M.nslices;
          sl += (long)gridDim.x * (kB / 64)) {
-        const int r = rowid[sl * 64 + lane];
+        const int r = M.rowid[sl * 64 + lane];
         if (r < 0) continue;
-        const int w = wid[sl];
-        const T *v = val + vptr[sl] + lane;
-        const int *c = col + vptr[sl] + lane;
-        // four independent chains keep four gathers of x in flight per lane
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int k = 0;
-        for (; k + 3 < w; k += 4) {
-            s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
-            s1 += (double)ld<NT>(v + (k + 1) * 64) * (double)xin[ld<NT>(c + (k + 1) * 64)];
-            s2 += (double)ld<NT>(v + (k + 2) * 64) * (double)xin[ld<NT>(c + (k + 2) * 64)];
-            s3 += (double)ld<NT>(v + (k + 3) * 64) * (double)xin[ld<NT>(c + (k + 3) * 64)];
-        }
-        for (; k < w; ++k) s0 += (double)ld<NT>(v + k * 64) * (double)xin[ld<NT>(c + k * 64)];
-        xout[r] = (W)((double)b[r] - ((s0 + s1) + (s2 + s3)));
+        const double s = sell_slice_row<NT>(M, val, sl, lane, xin);
+        xout[r] = (W)((double)b[r] - s);
     }
 }
 
 template <typename T, typename W>
-__global__ __launch_bounds__(kB) void k_sub_sell(int nslices, const long *__restrict__ vptr,
-                                                 const int *__restrict__ wid,
-                                                 const int *__restrict__ rowid,
-                                                 const T *__restrict__ val,
-                                                 const int *__restrict__ col,
+__global__ __launch_bounds__(kB) void k_sub_sell(SellView M, const T *__restrict__ val,
                                                  const W *__restrict__ x, W *__restrict__ y,
                                                  int xcd) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < nslices;
+    for (long sl = logical_block(xcd) * (kB / 64) + wv; sl < M.nslices;
          sl += (long)gridDim.x * (kB / 64)) {
-        const int r = rowid[sl * 64 + lane];
+        const int r = M.rowid[sl * 64 + lane];
         if (r < 0) continue;
-        const int w = wid[sl];
-        const T *v = val + vptr[sl] + lane;
-        const int *c = col + vptr[sl] + lane;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int k = 0;
-        for (; k + 3 < w; k += 4) {
-            s0 += (double)v[k * 64] * (double)x[c[k * 64]];
-            s1 += (double)v[(k + 1) * 64] * (double)x[c[(k + 1) * 64]];
-            s2 += (double)v[(k + 2) * 64] * (double)x[c[(k + 2) * 64]];
-            s3 += (double)v[(k + 3) * 64] * (double)x[c[(k + 3) * 64]];
-        }
-        for (; k < w; ++k) s0 += (double)v[k * 64] * (double)x[c[k * 64]];
-        y[r] = (W)((double)y[r] - ((s0 + s1) + (s2 + s3)));
+        const double s = sell_slice_row<false>(M, val, sl, lane, x);
+        y[r] = (W)((double)y[r] - s);
     }
 }
 
@@ -223,24 +222,19 @@ void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, 
                                                                    x);
 }
 template <typename T, typename W>
-void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                            const int *rowid, const T *val, const int *col, const W *b,
+void ildl_jacobi_sweep_sell(hipStream_t s, const SellView &M, const T *val, const W *b,
                             const W *xin, W *xout, int xcd) {
-    if (nslices <= 0) return;
+    if (M.nslices <= 0) return;
+    const int g = blocks_for(M.nslices, kB / 64, 1 << 20);
     if (xcd & 2)
-        k_jacobi_sell<T, W, true><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
-            nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd & 1);
+        k_jacobi_sell<T, W, true><<<g, kB, 0, s>>>(M, val, b, xin, xout, xcd & 1);
     else
-        k_jacobi_sell<T, W, false><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(
-            nslices, vptr, wid, rowid, val, col, b, xin, xout, xcd & 1);
+        k_jacobi_sell<T, W, false><<<g, kB, 0, s>>>(M, val, b, xin, xout, xcd & 1);
 }
 template <typename T, typename W>
-void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const W *x, W *y, int xcd) {
-    if (nslices <= 0) return;
-    k_sub_sell<T, W><<<blocks_for(nslices, kB / 64, 1 << 20), kB, 0, s>>>(nslices, vptr, wid,
-                                                                         rowid, val, col, x, y,
-                                                                         xcd);
+void ildl_sub_sell(hipStream_t s, const SellView &M, const T *val, const W *x, W *y, int xcd) {
+    if (M.nslices <= 0) return;
+    k_sub_sell<T, W><<<blocks_for(M.nslices, kB / 64, 1 << 20), kB, 0, s>>>(M, val, x, y, xcd);
 }
 template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
@@ -284,12 +278,10 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y) {
 #define INST(T, W)                                                                           \
     template void ildl_trsv_level<T, W>(hipStream_t, const int *, int, const int *,          \
                                         const int *, const T *, W *);                        \
-    template void ildl_jacobi_sweep_sell<T, W>(hipStream_t, int, const long *, const int *,   \
-                                               const int *, const T *, const int *,          \
+    template void ildl_jacobi_sweep_sell<T, W>(hipStream_t, const SellView &, const T *,      \
                                                const W *, const W *, W *, int);              \
-    template void ildl_sub_sell<T, W>(hipStream_t, int, const long *, const int *,           \
-                                      const int *, const T *, const int *, const W *, W *,   \
-                                      int);                                                  \
+    template void ildl_sub_sell<T, W>(hipStream_t, const SellView &, const T *, const W *,    \
+                                      W *, int);                                             \
     template void ildl_jacobi_csr<T, W>(hipStream_t, int, int, const int *, const int *,      \
                                         const T *, const W *, const W *, W *);               \
     template void ildl_sub_csr<T, W>(hipStream_t, int, int, const int *, const int *,         \

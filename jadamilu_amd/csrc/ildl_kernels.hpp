@@ -24,15 +24,26 @@ void ildl_scatter(hipStream_t s, int n, int nB, const int *perm, const double *s
 template <typename T, typename W>
 void ildl_trsv_level(hipStream_t s, const int *rows, int count, const int *ptr, const int *idx,
                      const T *val, W *x);
-// One Jacobi sweep of the unit triangle, xout = b - M xin, on a SELL-C-sigma
-// copy of M: slices of 64 rows (one wave64, one lane per row) taken from
-// windows of rows sorted by length, entries of a slice stored column-major
-// (value and column at vptr[s] + k*64 + lane, k < wid[s]); rowid[s*64 +
-// lane] is the row a lane owns (-1 for padding lanes).  xcd bit 0:
+// Device view of a SELL-C-sigma matrix: slices of 64 rows (one wave64, one
+// lane per row) taken from windows of rows sorted by length; entries of a
+// slice stored column-major (value at val[vptr[s] + k*64 + lane], k <
+// width); the slice's column indices start at byte iptr[s] of idx, as
+// 16-bit offsets from cbase[s] when code[s] is odd, else as int32 columns;
+// width = code[s] >> 1; rowid[s*64 + lane] is the row a lane owns (-1 for
+// padding lanes).
+struct SellView {
+    int nslices = 0;
+    const long *vptr = nullptr;
+    const long *iptr = nullptr;
+    const int *code = nullptr;
+    const int *rowid = nullptr;
+    const int *cbase = nullptr;
+    const char *idx = nullptr;
+};
+// One Jacobi sweep of the unit triangle, xout = b - M xin.  xcd bit 0:
 // XCD-contiguous block order; bit 1: nontemporal loads of the matrix stream.
 template <typename T, typename W>
-void ildl_jacobi_sweep_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                            const int *rowid, const T *val, const int *col, const W *b,
+void ildl_jacobi_sweep_sell(hipStream_t s, const SellView &M, const T *val, const W *b,
                             const W *xin, W *xout, int xcd);
 // CSR forms with lpr (4, 16 or 64) lanes per row, for matrices with long rows
 template <typename T, typename W>
@@ -43,8 +54,7 @@ void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *
                   const T *val, const W *x, W *y);
 // y[r] -= sum_j M(r, j) x[j] over the rows of a SELL-C-sigma matrix
 template <typename T, typename W>
-void ildl_sub_sell(hipStream_t s, int nslices, const long *vptr, const int *wid,
-                   const int *rowid, const T *val, const int *col, const W *x, W *y, int xcd);
+void ildl_sub_sell(hipStream_t s, const SellView &M, const T *val, const W *x, W *y, int xcd);
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x);
 // y = Minv x, Minv m x m row-major

@@ -65,6 +65,53 @@ def test_device_apply_vs_oracle(gen, shift, spd, single):
     ctx.close()
 
 
+def _far(n, lo, hi, seed=3):
+    """Diagonally dominant SPD matrix: a path plus n/5 couplings lo..hi rows
+    apart (lo >= 65536 puts factor columns more than 2^16 apart into one
+    SELL slice: 32-bit column slices)."""
+    rng = np.random.default_rng(seed)
+    r = np.arange(n - 1)
+    i = rng.integers(0, n - hi, n // 5)
+    rows = np.concatenate([r, i])
+    cols = np.concatenate([r + 1, i + rng.integers(lo, hi, i.size)])
+    key = np.unique(rows.astype(np.int64) * n + cols)
+    rows, cols = key // n, key % n
+    return M.from_pairs(n, rows, cols, -0.5 + 0.1 * rng.standard_normal(rows.size), 4.0)
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("form", [0, 4, 16, 64])
+@pytest.mark.parametrize("gen,spd", [
+    (lambda: M.anderson(14, 16.5, 4), False),
+    (lambda: _far(140001, 66000, 70000), True),
+])
+def test_every_kernel_form_matches_oracle(gen, spd, form, single):
+    """Each kernel form forced on every product of the factor (0: one lane
+    per row on SELL-C-sigma, with 16-bit column slices on the Anderson factor
+    and 32-bit ones on the far-coupled one; 4/16/64: CSR lane groups)
+    reproduces the oracle's multilevel apply, in exact mode and in sweep
+    mode (3 Jacobi sweeps per triangular solve on both sides)."""
+    from jadamilu_amd.ildl import IldlFactor
+    from jadamilu_amd.jd import Context
+    _dev()
+    ia, ja, a = gen()
+    n = ia.size - 1
+    f = IldlFactor(n, ia, ja, a, droptol=1e-3, spd=spd, single=single, dense_max=300)
+    assert f.ierr == 0
+    ex = f.export()
+    ctx = Context(0)
+    ctx.set_option("ildl_form", form)
+    r = np.random.default_rng(7).standard_normal(n)
+    tol = 2e-5 if single else 1e-12
+    for k in (0, 3):
+        ctx.attach_ildl(f, sweeps=k)
+        assert all(fm == [form] * 4 for fm in ctx.ildl_level_forms())
+        z = _apply_dev(ctx, r)
+        zo = O.ildl_apply(ex, r, sweeps=k)
+        assert np.linalg.norm(z - zo) <= tol * np.linalg.norm(zo), (k, form)
+    ctx.close()
+
+
 @pytest.mark.parametrize("shape", [(12, 11, 10), (48, 40, 36)])
 def test_sweep_mode_converges_to_exact(shape):
     """Jacobi sweeps reach the exact apply after as many sweeps as dependency
