@@ -151,12 +151,21 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
  * reference's behaviour, except "ildl_sweeps"): "ildl_sweeps" (0 = exact
  * level-scheduled triangular solves, k > 0 = k Jacobi sweeps per triangular
  * solve, -1 = the default: chosen at set-up by timing both on the device,
- * sweeps only where they reproduce the exact apply to 3 %),
- * "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum
- * degree, the default), "ildl_dense_max"
- * (largest final level factored densely), "spmv_lanes", "ildl_graph" (1:
- * replay the preconditioner apply from captured HIP graphs; default 0),
- * "ildl_storage", "ildl_xcd", "ildl_sigma" (device layout of the factor).  0 on success. */
+ * sweeps only where they reproduce the exact apply to "ildl_tune_tol", by
+ * default 3e-2 for definite and 1e-4 for indefinite factors),
+ * "ildl_autotune" (1 = kernel form of each factor product chosen by timing,
+ * the default), "ildl_form" (-1 = automatic; 0, 4, 16, 64 = that form for
+ * every product), "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee,
+ * 2 approximate minimum degree, the default), "ildl_dense_max" (largest final
+ * level factored densely), "spmv_lanes", "ildl_graph" (1: replay the
+ * preconditioner apply from captured HIP graphs; default 0), "ildl_storage",
+ * "ildl_xcd", "ildl_sigma" (device layout of the factor), "ildl_threads"
+ * (host threads of the factorization), and for ICNTL(5) != 0: "ildl_vec32"
+ * (work vectors of the apply: 1 FP32, 0 FP64, -1 FP32 for definite factors
+ * only, the default), "inner_fp32" (1 = FP32 inner solver; default 0: FP64
+ * inner solver as in the reference), "inner_spmv64" (FP32 inner solver: 1 =
+ * its SpMV reads the FP64 values, the default).  0 on success, -2 for an
+ * unknown name, -3 for an invalid value. */
 int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value);
 
 /* z = P^-1 r with the preconditioner currently held by ctx (diagonal or
