@@ -85,71 +85,9 @@ static double pow2_near(double v) {
     return std::exp2(std::round(std::log2(v)));
 }
 
-// Reverse Cuthill-McKee ordering (George & Liu), one BFS per component from
-// a pseudo-peripheral node; returns order[new] = old.
+// Reverse Cuthill-McKee ordering of the pattern (ordering.cpp)
 static std::vector<int> rcm_order(const SymCsr &A) {
-    const int n = A.n;
-    std::vector<int> deg(n), order;
-    order.reserve(n);
-    for (int i = 0; i < n; ++i) deg[i] = A.ptr[i + 1] - A.ptr[i];
-    std::vector<char> seen(n, 0);
-    std::vector<int> lvl(n, -1), q;
-    q.reserve(n);
-    auto bfs_far = [&](int s, int &far, int &depth) {
-        q.clear();
-        q.push_back(s);
-        lvl[s] = 0;
-        size_t h = 0;
-        far = s;
-        depth = 0;
-        while (h < q.size()) {
-            int u = q[h++];
-            if (lvl[u] > depth || (lvl[u] == depth && deg[u] < deg[far])) {
-                depth = lvl[u];
-                far = u;
-            }
-            for (int k = A.ptr[u]; k < A.ptr[u + 1]; ++k) {
-                int v = A.idx[k];
-                if (lvl[v] < 0) {
-                    lvl[v] = lvl[u] + 1;
-                    q.push_back(v);
-                }
-            }
-        }
-        for (int u : q) lvl[u] = -1;
-    };
-    std::vector<int> nb;
-    for (int start = 0; start < n; ++start) {
-        if (seen[start]) continue;
-        // pseudo-peripheral node: repeat BFS from the farthest node
-        int s = start, far, depth, last = -1;
-        for (int it = 0; it < 4; ++it) {
-            bfs_far(s, far, depth);
-            if (depth <= last) break;
-            last = depth;
-            s = far;
-        }
-        size_t head = order.size();
-        order.push_back(s);
-        seen[s] = 1;
-        while (head < order.size()) {
-            int u = order[head++];
-            nb.clear();
-            for (int k = A.ptr[u]; k < A.ptr[u + 1]; ++k) {
-                int v = A.idx[k];
-                if (!seen[v]) {
-                    seen[v] = 1;
-                    nb.push_back(v);
-                }
-            }
-            std::sort(nb.begin(), nb.end(), [&](int x, int y) {
-                return deg[x] < deg[y] || (deg[x] == deg[y] && x < y);
-            });
-            order.insert(order.end(), nb.begin(), nb.end());
-        }
-    }
-    std::reverse(order.begin(), order.end());
-    return order;
+    return rcm_order(A.n, A.ptr.data(), A.idx.data());
 }
 
 namespace {

@@ -9,4 +9,9 @@ namespace jdhip {
 // order[new] = old.
 std::vector<int> amd_order(int n, const int *ptr, const int *idx);
 
+// Reverse Cuthill-McKee ordering of the same kind of pattern (George & Liu;
+// one BFS per connected component from a pseudo-peripheral node).  Returns
+// order[new] = old.
+std::vector<int> rcm_order(int n, const int *ptr, const int *idx);
+
 }  // namespace jdhip
