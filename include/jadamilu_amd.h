@@ -154,7 +154,7 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
  * sweeps only where they reproduce the exact apply to "ildl_tune_tol", by
  * default 3e-2 for definite and 1e-4 for indefinite factors),
  * "ildl_autotune" (1 = kernel form of each factor product chosen by timing,
- * the default), "ildl_form" (-1 = automatic; 0, 4, 16, 64 = that form for
+ * the default), "ildl_form" (-1 = automatic; 0, 4, 8, 16, 32, 64 = that form for
  * every product), "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee,
  * 2 approximate minimum degree, the default), "ildl_dense_max" (largest final
  * level factored densely), "spmv_lanes", "ildl_graph" (1: replay the

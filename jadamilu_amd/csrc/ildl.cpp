@@ -428,7 +428,7 @@ class IldlImpl : public IldlImplBase {
 
     // Kernel form of every product of every level (both sweeps, L_CB and
     // L_CB^T) by measurement: the fastest of one lane per row on the
-    // SELL-C-sigma copy and CSR with 4, 16 or 64 lanes per row, each timed
+    // SELL-C-sigma copy and CSR with 4, 8, 16, 32 or 64 lanes per row, each timed
     // over three launches on zeroed vectors.  Replaces the row-length
     // heuristic the upload starts from.
     void autotune_forms() override {
@@ -446,7 +446,7 @@ class IldlImpl : public IldlImplBase {
             JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             return ms;
         };
-        const int forms[4] = {0, 4, 16, 64};
+        const int forms[6] = {0, 4, 8, 16, 32, 64};
         auto best = [&](auto &&launch_form) {
             int bf = 0;
             float bt = 0.f;

@@ -1,7 +1,7 @@
 // ildl_kernels.hip -- gfx950 kernels of the multilevel ILDL apply
 // (see ildl_kernels.hpp).  Products with short rows run one lane per row
 // on SELL-C-sigma copies (every load of a wave contiguous); long rows run
-// on CSR with lane groups of 4/16/64 and a butterfly reduction; the exact
+// on CSR with lane groups of 4 to 64 lanes and a butterfly reduction; the exact
 // level-scheduled solve uses 4-lane groups per row of a level.  Every loop
 // is grid-stride; vector entries are gathered through the column index.
 // The factor's values are stored and streamed as T (double, or float on
@@ -240,29 +240,29 @@ template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
                      const T *val, const W *b, const W *xin, W *xout) {
     if (nrows <= 0) return;
-    if (lpr >= 64)
-        k_jacobi_csr<T, W, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(
-            nrows, ptr, idx, val, b, xin, xout);
-    else if (lpr >= 16)
-        k_jacobi_csr<T, W, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(
-            nrows, ptr, idx, val, b, xin, xout);
-    else
-        k_jacobi_csr<T, W, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(
-            nrows, ptr, idx, val, b, xin, xout);
+#define JDHIP_JC(L)                                                                     \
+    k_jacobi_csr<T, W, L><<<blocks_for(nrows, kB / L, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, \
+                                                                            val, b, xin, xout)
+    if (lpr >= 64) JDHIP_JC(64);
+    else if (lpr >= 32) JDHIP_JC(32);
+    else if (lpr >= 16) JDHIP_JC(16);
+    else if (lpr >= 8) JDHIP_JC(8);
+    else JDHIP_JC(4);
+#undef JDHIP_JC
 }
 template <typename T, typename W>
 void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
                   const T *val, const W *x, W *y) {
     if (nrows <= 0) return;
-    if (lpr >= 64)
-        k_sub_csr<T, W, 64><<<blocks_for(nrows, kB / 64, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
-                                                                              val, x, y);
-    else if (lpr >= 16)
-        k_sub_csr<T, W, 16><<<blocks_for(nrows, kB / 16, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
-                                                                              val, x, y);
-    else
-        k_sub_csr<T, W, 4><<<blocks_for(nrows, kB / 4, 1 << 20), kB, 0, s>>>(nrows, ptr, idx,
-                                                                            val, x, y);
+#define JDHIP_SC(L)                                                                      \
+    k_sub_csr<T, W, L><<<blocks_for(nrows, kB / L, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val, \
+                                                                         x, y)
+    if (lpr >= 64) JDHIP_SC(64);
+    else if (lpr >= 32) JDHIP_SC(32);
+    else if (lpr >= 16) JDHIP_SC(16);
+    else if (lpr >= 8) JDHIP_SC(8);
+    else JDHIP_SC(4);
+#undef JDHIP_SC
 }
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x) {

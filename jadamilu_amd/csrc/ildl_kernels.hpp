@@ -45,7 +45,7 @@ struct SellView {
 template <typename T, typename W>
 void ildl_jacobi_sweep_sell(hipStream_t s, const SellView &M, const T *val, const W *b,
                             const W *xin, W *xout, int xcd);
-// CSR forms with lpr (4, 16 or 64) lanes per row, for matrices with long rows
+// CSR forms with lpr (4, 8, 16, 32 or 64) lanes per row, for matrices with long rows
 template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
                      const T *val, const W *b, const W *xin, W *xout);
