@@ -134,10 +134,11 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
     from jadamilu_amd import matrices as M
     count, lmin, lmax, wmin, wmax, seed = params
     rng = np.random.default_rng(seed)
-    probs, ntot = [], 0
+    probs, meta, ntot = [], [], 0
     for _ in range(count):
         L = int(rng.integers(lmin, lmax + 1))
         W = float(rng.uniform(wmin, wmax))
+        meta.append((L, W))
         ia, ja, a = M.anderson(L, W, int(rng.integers(0, 1 << 30)))
         ntot += ia.size - 1
         probs.append(dict(ia=ia, ja=ja, a=a, neig=neig, isearch=isearch, sigma=sigma,
@@ -153,6 +154,7 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
     # with a progress line per chunk
     order = sorted(range(count), key=lambda i: -(probs[i]["ia"].size))
     chunks = [[probs[i] for i in order[c:c + 64]] for c in range(0, count, 64)]
+    failed = []
     for k in range(args.steps):
         out = []
         for ci, ch in enumerate(chunks):
@@ -161,6 +163,10 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
                   flush=True)
         got += sum(r.neig for r in out)
         fails += sum(r.info != 0 for r in out)
+        if k == 0:  # out follows `order`
+            failed = [{"L": meta[i][0], "W": round(meta[i][1], 3), "info": r.info,
+                       "neig": r.neig, "matvecs": r.iter}
+                      for i, r in zip(order, out) if r.info != 0]
         print(f"# batch step {k}: {sum(r.neig for r in out)} eigenpairs, "
               f"{sum(r.info != 0 for r in out)} failed items", file=sys.stderr, flush=True)
     el = time.perf_counter() - t0
@@ -169,14 +175,14 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
         "value": got / el, "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / max(args.steps, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 inner/factor + f64 outer", "data": "synthetic",
+        "dtype": "f32 factor + f64 inner/outer", "data": "synthetic",
         "config": {"workload": args.workload, "instances": count, "total_n": ntot,
                    "neig": neig, "tol": args.tol, "isearch": isearch, "sigma": sigma,
                    "workers": args.workers,
                    "precond": f"multilevel ILDL droptol {args.droptol}, FP32 (host factor "
                               f"per instance inside the timed region)"},
         "roofline": None, "cpu_baseline": None,
-        "result": {"failed_items": fails, "eigenpairs": got}}))
+        "result": {"failed_items": fails, "eigenpairs": got, "failed": failed}}))
 
 
 def bench_adjoint(args, params, neig, madspace, isearch, sigma, mem):

@@ -40,9 +40,7 @@ __global__ __launch_bounds__(kBlock) void k_finish_columns(const double *__restr
                                                           int nparts, double *out) {
     __shared__ double red[kBlock / 64];
     const int c = blockIdx.x, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) s += partials[(long)c * nparts + b];
-    s = wave_sum(s);
+    const double s = wave_sum(thread_partials(partials + (long)c * nparts, nparts));
     if (lane == 0) red[wid] = s;
     __syncthreads();
     if (threadIdx.x == 0) {

@@ -1,9 +1,9 @@
 // reduce.hpp -- deterministic grid-wide reductions for gfx950 kernels.
 //
 // xor-butterfly inside the wave64, LDS across the waves of the block,
-// per-block partials in global memory; the last block to take a ticket from
-// a device counter sums the partials in a fixed order, so results do not
-// depend on scheduling.  Accumulation is always in double.
+// per-block partials in global memory, summed in a fixed order by a second
+// (finish) launch, so results do not depend on scheduling.  Accumulation is
+// always in double.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -15,6 +15,22 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// This thread's share of a sum of nparts partials read by one block, in a
+// fixed order.  Four independent chains: the partials come from other XCDs'
+// writes (memory latency per load), so several loads stay in flight.
+__device__ __forceinline__ double thread_partials(const double *__restrict__ pc, int nparts) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * kBlock < nparts; b += 4 * kBlock) {
+        s0 += pc[b];
+        s1 += pc[b + kBlock];
+        s2 += pc[b + 2 * kBlock];
+        s3 += pc[b + 3 * kBlock];
+    }
+    for (; b < nparts; b += kBlock) s0 += pc[b];
+    return (s0 + s1) + (s2 + s3);
 }
 
 // Two-launch deterministic reduction.  Partial launch (nparts == 0): every
@@ -49,9 +65,7 @@ __device__ bool grid_reduce(double (&v)[NC], int nc, double *__restrict__ partia
     }
     __shared__ double red[kBlock / 64];
     for (int c = 0; c < nc; ++c) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nparts; b += kBlock) s += partials[(long)c * nparts + b];
-        s = wave_sum(s);
+        double s = wave_sum(thread_partials(partials + (long)c * nparts, nparts));
         if (lane == 0) red[wid] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
