@@ -154,7 +154,7 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
  * sweeps only where they reproduce the exact apply to "ildl_tune_tol", by
  * default 3e-2 for definite and 1e-4 for indefinite factors),
  * "ildl_autotune" (1 = kernel form of each factor product chosen by timing,
- * the default), "ildl_form" (-1 = automatic; 0, 4, 8, 16, 32, 64 = that form for
+ * the default), "ildl_form" (-1 = automatic; 0, 1, 4, 8, 16, 32, 64 = that form for
  * every product), "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee,
  * 2 approximate minimum degree, the default), "ildl_dense_max" (largest final
  * level factored densely), "spmv_lanes", "ildl_graph" (1: replay the
@@ -206,7 +206,8 @@ int jdhip_ildl_device_info(jdhip_ctx *ctx, int *fwd_levels, int *bwd_levels, lon
 int jdhip_ildl_level_sweeps(jdhip_ctx *ctx, int *out, int cap);
 /* Kernel form of each product of each sparse level, four per level (forward
  * sweep over L_B, backward sweep over L_B^T, L_CB product, L_CB^T product):
- * 0 one lane per row on a SELL-C-sigma copy, k > 0 CSR with k lanes per row.
+ * 0 one lane per row on a SELL-C-sigma copy, 1 CSR-stream (row blocks of at
+ * most 2048 entries staged through LDS), k >= 4 CSR with k lanes per row.
  * Chosen by timing at set-up (option ildl_autotune, default 1), else by the
  * average row length.  Fills out[0..cap-1]; returns the number of levels, or
  * -1 without an ILDL preconditioner.  No reference counterpart. */

@@ -971,7 +971,8 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_tune_tol") S.ildl_tune_tol = value;
     else if (k == "ildl_form") {
         const int f = (int)value;
-        if (f != -1 && f != 0 && f != 4 && f != 8 && f != 16 && f != 32 && f != 64) return -3;
+        if (f != -1 && f != 0 && f != 1 && f != 4 && f != 8 && f != 16 && f != 32 && f != 64)
+            return -3;
         S.ildl_form = f;
     }
     else if (k == "ildl_graph") {

@@ -203,6 +203,9 @@ struct DevLevel {
     T *lc_val = nullptr;
     int *lct_ptr = nullptr, *lct_idx = nullptr;
     T *lct_val = nullptr;
+    // row blocks of the CSR-stream form for L_B, L_B^T, L_CB, L_CB^T
+    int *rb_b = nullptr, *rb_t = nullptr, *rb_c = nullptr, *rb_ct = nullptr;
+    int nrb_b = 0, nrb_t = 0, nrb_c = 0, nrb_ct = 0;
     std::vector<int> fwd_ptr, bwd_ptr;  // host level pointers
     int *fwd_rows = nullptr, *bwd_rows = nullptr;
     W *buf = nullptr, *out = nullptr, *t1 = nullptr, *t2 = nullptr;  // work vectors
@@ -304,6 +307,8 @@ class IldlImpl : public IldlImplBase {
             D.lt_ptr = track(dev_upload(tp));
             D.lt_idx = track(dev_upload(ti));
             D.lt_val = track(dev_upload(tv));
+            upload_row_blocks(nB, bp, D.rb_b, D.nrb_b);
+            upload_row_blocks(nB, tp, D.rb_t, D.nrb_t);
             D.sf = upload_sell(build_sell(nB, bp, bi, bv, sigma_));
             D.sb = upload_sell(build_sell(nB, tp, ti, tv, sigma_));
             // dependency levels from the triangular (factor) numbering
@@ -322,6 +327,7 @@ class IldlImpl : public IldlImplBase {
             D.lc_ptr = track(dev_upload(cp));
             D.lc_idx = track(dev_upload(ci));
             D.lc_val = track(dev_upload(cv));
+            upload_row_blocks(nC, cp, D.rb_c, D.nrb_c);
             D.wc = pick_lanes((long)ci.size(), nC);
             D.wf = D.wb = pick_lanes((long)bi.size(), nB);
             transpose(nC, nB, cp, ci, cv, tp, ti, tv);
@@ -329,6 +335,7 @@ class IldlImpl : public IldlImplBase {
             D.lct_ptr = track(dev_upload(tp));
             D.lct_idx = track(dev_upload(ti));
             D.lct_val = track(dev_upload(tv));
+            upload_row_blocks(nB, tp, D.rb_ct, D.nrb_ct);
             D.wct = pick_lanes((long)ti.size(), nB);
             D.buf = track(dev_alloc<W>(n));
             D.out = track(dev_alloc<W>(n));
@@ -403,7 +410,11 @@ class IldlImpl : public IldlImplBase {
     void sweep_launch(const DevLevel<T, W> &D, bool fwd, int form, const W *b, const W *xin,
                       W *xout) {
         hipStream_t s = ctx_->stream;
-        if (form > 0) {
+        if (form == 1) {
+            ildl_jacobi_stream<T, W>(s, fwd ? D.nrb_b : D.nrb_t, fwd ? D.rb_b : D.rb_t,
+                                     fwd ? D.lb_ptr : D.lt_ptr, fwd ? D.lb_idx : D.lt_idx,
+                                     fwd ? D.lb_val : D.lt_val, b, xin, xout);
+        } else if (form > 0) {
             ildl_jacobi_csr<T, W>(s, form, D.nB, fwd ? D.lb_ptr : D.lt_ptr,
                                   fwd ? D.lb_idx : D.lt_idx, fwd ? D.lb_val : D.lt_val, b, xin,
                                   xout);
@@ -415,7 +426,13 @@ class IldlImpl : public IldlImplBase {
     // y -= L_CB x (C rows; trans = false) or y -= L_CB^T x (B rows)
     void sub_launch(const DevLevel<T, W> &D, bool trans, int form, const W *x, W *y) {
         hipStream_t s = ctx_->stream;
-        if (form > 0) {
+        if (form == 1) {
+            if (trans)
+                ildl_sub_stream<T, W>(s, D.nrb_ct, D.rb_ct, D.lct_ptr, D.lct_idx, D.lct_val, x,
+                                      y);
+            else
+                ildl_sub_stream<T, W>(s, D.nrb_c, D.rb_c, D.lc_ptr, D.lc_idx, D.lc_val, x, y);
+        } else if (form > 0) {
             if (trans)
                 ildl_sub_csr<T, W>(s, form, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, x, y);
             else
@@ -428,7 +445,8 @@ class IldlImpl : public IldlImplBase {
 
     // Kernel form of every product of every level (both sweeps, L_CB and
     // L_CB^T) by measurement: the fastest of one lane per row on the
-    // SELL-C-sigma copy and CSR with 4, 8, 16, 32 or 64 lanes per row, each timed
+    // SELL-C-sigma copy, the CSR-stream form (form 1) and CSR with 4, 8, 16,
+    // 32 or 64 lanes per row, each timed
     // over three launches on zeroed vectors.  Replaces the row-length
     // heuristic the upload starts from.
     void autotune_forms() override {
@@ -446,7 +464,7 @@ class IldlImpl : public IldlImplBase {
             JDHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             return ms;
         };
-        const int forms[6] = {0, 4, 8, 16, 32, 64};
+        const int forms[7] = {0, 1, 4, 8, 16, 32, 64};
         auto best = [&](auto &&launch_form) {
             int bf = 0;
             float bt = 0.f;
@@ -491,6 +509,12 @@ class IldlImpl : public IldlImplBase {
             v.push_back(D.wct);
         }
         return v;
+    }
+
+    void upload_row_blocks(int nrows, const std::vector<int> &ptr, int *&dev, int &nblk) {
+        const std::vector<int> rb = stream_row_blocks(nrows, ptr.data());
+        nblk = (int)rb.size() - 1;
+        dev = track(dev_upload(rb));
     }
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
@@ -713,7 +737,7 @@ class IldlImpl : public IldlImplBase {
     }
     // values and column indices one product streams in the given form
     static long mat_bytes(const SellDev<T> &M, int form, long nnz) {
-        if (form > 0) return ((long)sizeof(T) + 4) * nnz;
+        if (form > 0) return ((long)sizeof(T) + 4) * nnz;  // CSR (lanes or stream)
         return (long)sizeof(T) * nnz + 2 * M.nnz16 + 4 * M.nnz32;
     }
     long apply_bytes() const override {

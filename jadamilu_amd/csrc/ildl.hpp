@@ -59,10 +59,11 @@ class IldlImplBase {
     virtual std::vector<int> level_sweeps() const = 0;
     // Choose every product's kernel form (SELL or CSR lanes) by timing.
     virtual void autotune_forms() = 0;
-    // Set every product's form (0 SELL, 4/8/16/32/64 CSR lanes per row).
+    // Set every product's form (0 SELL, 1 CSR-stream, 4/8/16/32/64 CSR lanes
+    // per row).
     virtual void set_forms(int form) = 0;
     // per level: forms of the forward sweep, backward sweep, L_CB, L_CB^T
-    // (0 SELL, k lanes per row CSR)
+    // (0 SELL, 1 CSR-stream, k >= 4 lanes per row CSR)
     virtual std::vector<int> level_forms() const = 0;
     // replay the apply from captured HIP graphs (default on)
     virtual void set_graphs(bool on) = 0;

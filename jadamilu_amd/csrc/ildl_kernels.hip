@@ -13,6 +13,7 @@
 #include "ildl_kernels.hpp"
 
 #include <algorithm>
+#include <vector>
 
 namespace jdhip {
 
@@ -181,6 +182,99 @@ __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict
     }
 }
 
+// CSR-stream form: block b owns the rows rblk[b] .. rblk[b+1]-1, chosen on
+// the host so that their entries number at most kStreamCap (and rows at most
+// kB), or a single longer row.  The block streams the entries of its rows
+// (consecutive in CSR) with fully coalesced, independent loads, parks the
+// products in LDS, then sums each row with a group of t lanes (t = the power
+// of two that spreads the block's rows over its 256 threads, at most 64) and
+// a butterfly.  A single row longer than kStreamCap is summed by the whole
+// block with strided loads.  Every row of the range is written.
+static constexpr int kStreamCap = 2048;
+
+template <typename T, typename W>
+__device__ __forceinline__ void stream_rows(const int *__restrict__ rblk,
+                                            const int *__restrict__ ptr,
+                                            const int *__restrict__ idx,
+                                            const T *__restrict__ val,
+                                            const W *__restrict__ x, int &row, double &sum,
+                                            bool &owner) {
+    __shared__ double prod[kStreamCap];
+    __shared__ double red[kB / 64];
+    const int tid = threadIdx.x;
+    const int r0 = rblk[blockIdx.x], r1 = rblk[blockIdx.x + 1];
+    const int e0 = ptr[r0], e1 = ptr[r1];
+    const int ne = e1 - e0;
+    owner = false;
+    if (r1 - r0 == 1 && ne > kStreamCap) {  // one long row
+        double s0 = 0.0, s1 = 0.0;
+        int k = e0 + tid;
+        for (; k + kB < e1; k += 2 * kB) {
+            s0 += (double)val[k] * (double)x[idx[k]];
+            s1 += (double)val[k + kB] * (double)x[idx[k + kB]];
+        }
+        if (k < e1) s0 += (double)val[k] * (double)x[idx[k]];
+        double s = s0 + s1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < kB / 64; ++w) t += red[w];
+            row = r0;
+            sum = t;
+            owner = true;
+        }
+        return;
+    }
+    // ne <= kStreamCap and r1 - r0 <= kB by construction of rblk
+    for (int k = tid; k < ne; k += kB) prod[k] = (double)val[e0 + k] * (double)x[idx[e0 + k]];
+    __syncthreads();
+    const int nr = r1 - r0;
+    int t = 64;
+    while (t > 1 && t * nr > kB) t >>= 1;
+    const int g = tid / t, l = tid % t;
+    double s = 0.0;
+    if (g < nr) {
+        const int a = ptr[r0 + g] - e0, b = ptr[r0 + g + 1] - e0;
+        for (int k = a + l; k < b; k += t) s += prod[k];
+    }
+    for (int o = t / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, t);
+    if (g < nr && l == 0) {
+        row = r0 + g;
+        sum = s;
+        owner = true;
+    }
+}
+
+template <typename T, typename W>
+__global__ __launch_bounds__(kB) void k_jacobi_stream(const int *__restrict__ rblk,
+                                                      const int *__restrict__ ptr,
+                                                      const int *__restrict__ idx,
+                                                      const T *__restrict__ val, const W *b,
+                                                      const W *xin, W *xout) {
+    int row = 0;
+    double s = 0.0;
+    bool owner = false;
+    stream_rows(rblk, ptr, idx, val, xin, row, s, owner);
+    if (owner) xout[row] = (W)((double)b[row] - s);
+}
+
+template <typename T, typename W>
+__global__ __launch_bounds__(kB) void k_sub_stream(const int *__restrict__ rblk,
+                                                   const int *__restrict__ ptr,
+                                                   const int *__restrict__ idx,
+                                                   const T *__restrict__ val,
+                                                   const W *__restrict__ x, W *__restrict__ y) {
+    int row = 0;
+    double s = 0.0;
+    bool owner = false;
+    stream_rows(rblk, ptr, idx, val, x, row, s, owner);
+    if (owner) y[row] = (W)((double)y[row] - s);
+}
+
 template <typename W>
 __global__ __launch_bounds__(kB) void k_dscale(int n, const double *__restrict__ dinv, W *x) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
@@ -264,6 +358,42 @@ void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *
     else JDHIP_SC(4);
 #undef JDHIP_SC
 }
+template <typename T, typename W>
+void ildl_jacobi_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr,
+                        const int *idx, const T *val, const W *b, const W *xin, W *xout) {
+    if (nblk <= 0) return;
+    k_jacobi_stream<T, W><<<nblk, kB, 0, s>>>(rblk, ptr, idx, val, b, xin, xout);
+}
+template <typename T, typename W>
+void ildl_sub_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr, const int *idx,
+                     const T *val, const W *x, W *y) {
+    if (nblk <= 0) return;
+    k_sub_stream<T, W><<<nblk, kB, 0, s>>>(rblk, ptr, idx, val, x, y);
+}
+std::vector<int> stream_row_blocks(int nrows, const int *ptr) {
+    std::vector<int> rb;
+    rb.push_back(0);
+    int r = 0;
+    while (r < nrows) {
+        const int len = ptr[r + 1] - ptr[r];
+        if (len > kStreamCap) {  // a long row alone
+            ++r;
+            rb.push_back(r);
+            continue;
+        }
+        int e = 0, cnt = 0;
+        while (r < nrows && cnt < kB) {
+            const int l = ptr[r + 1] - ptr[r];
+            if (l > kStreamCap || e + l > kStreamCap) break;
+            e += l;
+            ++cnt;
+            ++r;
+        }
+        rb.push_back(r);
+    }
+    return rb;
+}
+
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x) {
     if (n <= 0) return;
@@ -286,7 +416,12 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y) {
                                         const T *, const W *, const W *, W *);               \
     template void ildl_sub_csr<T, W>(hipStream_t, int, int, const int *, const int *,         \
                                      const T *, const W *, W *);                             \
-    template void ildl_dense_gemv<T, W>(hipStream_t, int, const T *, const W *, W *);
+    template void ildl_dense_gemv<T, W>(hipStream_t, int, const T *, const W *, W *);         \
+    template void ildl_jacobi_stream<T, W>(hipStream_t, int, const int *, const int *,        \
+                                           const int *, const T *, const W *, const W *,      \
+                                           W *);                                              \
+    template void ildl_sub_stream<T, W>(hipStream_t, int, const int *, const int *,           \
+                                        const int *, const T *, const W *, W *);
 INST(double, double)
 INST(float, double)
 INST(float, float)

@@ -10,6 +10,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace jdhip {
 
 // Factor values are T (double or float), work vectors W (double or float);
@@ -55,6 +57,16 @@ void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *
 // y[r] -= sum_j M(r, j) x[j] over the rows of a SELL-C-sigma matrix
 template <typename T, typename W>
 void ildl_sub_sell(hipStream_t s, const SellView &M, const T *val, const W *x, W *y, int xcd);
+// CSR-stream form: blocks of consecutive rows with at most 2048 entries
+// (and 256 rows) each, or one longer row; rblk[b] = first row of block b,
+// rblk[nblk] = nrows.  Entries are streamed through LDS with coalesced loads.
+std::vector<int> stream_row_blocks(int nrows, const int *ptr);
+template <typename T, typename W>
+void ildl_jacobi_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr,
+                        const int *idx, const T *val, const W *b, const W *xin, W *xout);
+template <typename T, typename W>
+void ildl_sub_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr, const int *idx,
+                     const T *val, const W *x, W *y);
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x);
 // y = Minv x, Minv m x m row-major

@@ -276,7 +276,8 @@ class Context:
 
     def ildl_level_forms(self):
         """Per sparse level: kernel forms (forward sweep, backward sweep, L_CB,
-        L_CB^T); 0 SELL one lane per row, k CSR with k lanes per row."""
+        L_CB^T); 0 SELL one lane per row, 1 CSR-stream, k >= 4 CSR with k lanes
+        per row."""
         buf = (C.c_int * 256)()
         k = self._L.jdhip_ildl_level_forms(self.h, buf, 256)
         return [list(buf[4 * l: 4 * l + 4]) for l in range(max(k, 0))]

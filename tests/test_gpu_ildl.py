@@ -80,7 +80,7 @@ def _far(n, lo, hi, seed=3):
 
 
 @pytest.mark.parametrize("single", [False, True])
-@pytest.mark.parametrize("form", [0, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("form", [0, 1, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("gen,spd", [
     (lambda: M.anderson(14, 16.5, 4), False),
     (lambda: _far(140001, 66000, 70000), True),
@@ -88,7 +88,8 @@ def _far(n, lo, hi, seed=3):
 def test_every_kernel_form_matches_oracle(gen, spd, form, single):
     """Each kernel form forced on every product of the factor (0: one lane
     per row on SELL-C-sigma, with 16-bit column slices on the Anderson factor
-    and 32-bit ones on the far-coupled one; 4 to 64: CSR lane groups)
+    and 32-bit ones on the far-coupled one; 1: CSR-stream through LDS; 4 to
+    64: CSR lane groups)
     reproduces the oracle's multilevel apply, in exact mode and in sweep
     mode (3 Jacobi sweeps per triangular solve on both sides)."""
     from jadamilu_amd.ildl import IldlFactor
