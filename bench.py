@@ -98,7 +98,8 @@ def kernel_probe(ctx, n, nnz_u, reps, dev, torch, sweep=None):
         ctx.attach_ildl(f, sweeps=k)
         ms, nb = ctx.ildl_time_sweep(reps)
         out.update({"sweep_avg_ms": ms, "sweep_alg_bytes": nb,
-                    "sweeps_chosen": ctx.stats()["precond_sweeps"]})
+                    "sweeps_chosen": ctx.stats()["precond_sweeps"],
+                    "level_forms": ctx.ildl_level_forms()})
     xs = torch.ones(n, dtype=torch.float64, device=dev)
     ys = torch.empty(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
@@ -445,9 +446,17 @@ def main():
         # stream, same launches as inside the solve)
         sw_ms, sw_bytes = ctx.ildl_time_sweep(args.spmv_reps)
         sw_ach = sw_bytes / (sw_ms * 1e-3) / 1e9
+        form = pinfo["level_forms"][0][0] if pinfo and pinfo.get("level_forms") else 0
+        kname = {0: "k_jacobi_sell", 1: "k_jacobi_stream"}.get(form, f"k_jacobi_csr<{form}>")
+        kdesc = {0: "SELL-C-sigma, one lane per row",
+                 1: "CSR-stream, row blocks through LDS"}.get(form, f"CSR, {form} lanes per row")
+        # the traffic file holds counters of one kernel form; use them only
+        # when the set-up picked that form
         roofline = {"bound": "hbm", "achieved": sw_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": sw_ach / HBM_PEAK_GBS, "traffic": traffic.get("sweep"),
-                    "kernel": "k_jacobi_sell (ILDL level-1 Jacobi sweep, SELL-C-sigma)",
+                    "frac": sw_ach / HBM_PEAK_GBS,
+                    "traffic": (traffic.get("sweep")
+                                if traffic.get("sweep_kernel") == kname else None),
+                    "kernel": f"{kname} (ILDL level-1 Jacobi sweep, {kdesc})",
                     "alg_bytes": sw_bytes, "avg_ms": sw_ms}
 
     # ---- CPU baseline: oracle on a bounded sample of the same solve ----
