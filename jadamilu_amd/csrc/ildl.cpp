@@ -38,6 +38,25 @@ X *dev_alloc(size_t cnt) {
     return p;
 }
 
+// Sort the entries of every CSR row by column.
+template <typename T>
+void sort_rows(int nrows, const std::vector<int> &ptr, std::vector<int> &idx,
+               std::vector<T> &val) {
+    std::vector<std::pair<int, T>> row;
+    for (int r = 0; r < nrows; ++r) {
+        const int b = ptr[r], e = ptr[r + 1];
+        if (e - b < 2) continue;
+        row.clear();
+        for (int k = b; k < e; ++k) row.emplace_back(idx[k], val[k]);
+        std::sort(row.begin(), row.end(),
+                  [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (int k = b; k < e; ++k) {
+            idx[k] = row[k - b].first;
+            val[k] = row[k - b].second;
+        }
+    }
+}
+
 // CSR transpose (rows x cols) -> (cols x rows)
 template <typename T>
 void transpose(int rows, int cols, const std::vector<int> &ptr, const std::vector<int> &idx,
@@ -298,6 +317,11 @@ class IldlImpl : public IldlImplBase {
                 }
                 cp[c + 1] = (int)ci.size();
             }
+            // columns ascending inside every row: neighbouring lanes of the
+            // stream and lane-group kernels then gather from the same cache
+            // lines (the transposes built below inherit the order)
+            sort_rows(nB, bp, bi, bv);
+            sort_rows(nC, cp, ci, cv);
             D.lb_ptr = track(dev_upload(bp));
             D.lb_idx = track(dev_upload(bi));
             D.lb_val = track(dev_upload(bv));
