@@ -79,11 +79,24 @@ def _far(n, lo, hi, seed=3):
     return M.from_pairs(n, rows, cols, -0.5 + 0.1 * rng.standard_normal(rows.size), 4.0)
 
 
+def _arrow(n):
+    """Path plus a hub coupled to every node (hub diagonal 2n, SPD): the
+    first level's L_CB holds a row of more than 2048 entries (the
+    single-row blocks of the CSR-stream form)."""
+    r = np.arange(n - 2)
+    rows = np.concatenate([r, np.arange(n - 1)])
+    cols = np.concatenate([r + 1, np.full(n - 1, n - 1)])
+    diag = np.full(n, 4.0)
+    diag[-1] = 2.0 * n
+    return M.from_pairs(n, rows, cols, -np.ones(rows.size), diag)
+
+
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("form", [0, 1, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("gen,spd", [
     (lambda: M.anderson(14, 16.5, 4), False),
     (lambda: _far(140001, 66000, 70000), True),
+    (lambda: _arrow(6000), True),
 ])
 def test_every_kernel_form_matches_oracle(gen, spd, form, single):
     """Each kernel form forced on every product of the factor (0: one lane
@@ -91,7 +104,8 @@ def test_every_kernel_form_matches_oracle(gen, spd, form, single):
     and 32-bit ones on the far-coupled one; 1: CSR-stream through LDS; 4 to
     64: CSR lane groups)
     reproduces the oracle's multilevel apply, in exact mode and in sweep
-    mode (3 Jacobi sweeps per triangular solve on both sides)."""
+    mode (3 Jacobi sweeps per triangular solve on both sides); the arrow
+    matrix has a product row longer than one CSR-stream block."""
     from jadamilu_amd.ildl import IldlFactor
     from jadamilu_amd.jd import Context
     _dev()
