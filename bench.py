@@ -459,6 +459,30 @@ def main():
                     "kernel": f"{kname} (ILDL level-1 Jacobi sweep, {kdesc})",
                     "alg_bytes": sw_bytes, "avg_ms": sw_ms}
 
+    # ---- mixed path: eigenvalues against the FP64 path on the same input ----
+    vs64 = None
+    if args.single and args.precond == "ildl" and r.neig > 0:
+        X[:n].copy_(x0_dev)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter()
+        r64 = ctx.pjd(n, a, ja, ia, X.data_ptr(), lx, neig=neig, isearch=isearch, sigma=sigma,
+                      ninit=1, madspace=madspace, maxiter=args.maxiter, tol=args.tol,
+                      droptol=args.droptol, mem=mem, icntl=(0, 1, 0, 0, 0))
+        torch.cuda.synchronize()
+        print(f"# FP64 comparison solve: {time.perf_counter() - t64:.2f} s, {r64.iter} "
+              f"MATVECs, info {r64.info}", file=sys.stderr, flush=True)
+        if r64.info == 0 and r64.neig > 0:
+            e64 = np.asarray(r64.eigs)
+            near = [float(e64[np.argmin(np.abs(e64 - l))]) for l in r.eigs]
+            abs_err = [abs(l - m) for l, m in zip(r.eigs, near)]
+            vs64 = {"fp64_neig": int(r64.neig), "fp64_matvecs": int(r64.iter),
+                    "max_abs_err": float(max(abs_err)),
+                    "max_rel_err": float(max(e / max(abs(m), 1e-300)
+                                             for e, m in zip(abs_err, near))),
+                    "max_err_over_norm1": float(max(abs_err)) / norm1(ia, ja, a)}
+        else:
+            vs64 = {"fp64_neig": int(r64.neig), "fp64_info": int(r64.info)}
+
     # ---- CPU baseline: oracle on a bounded sample of the same solve ----
     cpu = None
     if not args.no_cpu:
@@ -547,6 +571,7 @@ def main():
                    "end_to_end_matvecs": r_setup.iter,
                    "host_factor_s": setup_pre_s,
                    "fill": r.stats.get("fill"), "precond_levels": r.stats.get("precond_levels"),
+                   "vs_fp64": vs64,
                    "ildl_device": pinfo},
     }
     print(json.dumps(out))
