@@ -133,3 +133,37 @@ def test_amd_ordering_is_a_fill_reducing_permutation():
         f_rcm = _chol_nnz(S, reverse_cuthill_mckee(S, symmetric_mode=True))
         f_nat = _chol_nnz(S, np.arange(n))
         assert f_amd < 0.7 * f_rcm and f_amd < 0.5 * f_nat, (shape, f_amd, f_rcm, f_nat)
+
+
+def _depth(lv):
+    """Longest dependency chain of a level's unit lower triangle L_B."""
+    nB, p, idx = lv["nB"], lv["lb_ptr"], lv["lb_idx"]
+    d = np.zeros(max(nB, 1), np.int64)
+    for i in range(nB):
+        cols = idx[p[i]:p[i + 1]]
+        d[i] = (d[cols].max() + 1) if cols.size else 0
+    return int(d.max()) if nB else 0
+
+
+@pytest.mark.parametrize("gen,spd", [(lambda: M.laplace((9, 8, 7)), True),
+                                     (lambda: M.anderson(6, 16.5, 3), False)])
+def test_oracle_sweep_mode_is_exact_beyond_the_depth(gen, spd):
+    """The oracle's Jacobi sweep mode (x <- b - N x from x = b, the checker
+    of the device sweeps): N is strictly triangular with longest chain
+    `depth` edges, so N^(depth+1) = 0 and k sweeps from x = b give the
+    Neumann sum up to N^k: `depth` sweeps reproduce the exact substitution
+    on every level, one sweep does not (starting from 0 instead of b, or an
+    extra or missing sweep, breaks one of the two on these factors)."""
+    ia, ja, a = gen()
+    n = ia.size - 1
+    f = ildl.IldlFactor(n, ia, ja, a, droptol=1e-3, spd=spd, dense_max=50)
+    assert f.ierr == 0
+    ex = f.export()
+    depth = max(_depth(lv) for lv in ex["levels"])
+    assert depth >= 2
+    r = np.random.default_rng(4).standard_normal(n)
+    z0 = O.ildl_apply(ex, r)
+    zk = O.ildl_apply(ex, r, sweeps=depth)
+    np.testing.assert_allclose(zk, z0, rtol=0, atol=1e-12 * np.abs(z0).max())
+    z1 = O.ildl_apply(ex, r, sweeps=1)
+    assert np.linalg.norm(z1 - z0) > 1e-6 * np.linalg.norm(z0)
