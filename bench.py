@@ -66,6 +66,17 @@ def build_matrix(kind, params):
     return M.anderson(L, W, seed)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def norm1(ia, ja, a):
     """||A||_1 of the symmetric matrix given by its 1-based upper CSR."""
     n = ia.size - 1
@@ -94,6 +105,13 @@ def kernel_probe(ctx, n, nnz_u, reps, dev, torch, sweep=None):
     if sweep is not None:
         from jadamilu_amd.ildl import IldlFactor
         ia, ja, a, spd, single, k = sweep
+        if k < 0:
+            # No tuning here: the tuning times the exact level-scheduled apply
+            # (about 14,000 launches per apply on the 128^3 Laplacian), and a
+            # host-side SIGSEGV inside the launch path was once observed in
+            # that phase under rocprofv3 --pmc; kernel profiles use a fixed
+            # count instead.
+            k = 4
         f = IldlFactor(n, ia, ja, a, shift=0.0, droptol=1e-3, spd=spd, single=single)
         ctx.attach_ildl(f, sweeps=k)
         ms, nb = ctx.ildl_time_sweep(reps)
@@ -393,7 +411,9 @@ def main():
     elapsed = time.perf_counter() - t0
     r = results[-1]
     ms_per_step = 1e3 * elapsed / max(args.steps, 1)
-    eig_per_s = (r.neig * args.steps) / elapsed
+    # only solves that ended with INFO = 0 count, each with the pairs it returned
+    eig_per_s = sum(x.neig for x in results if x.info == 0) / elapsed
+    failed_solves = sum(1 for x in results if x.info != 0)
     pre_s = sum(x.stats.get("precond_seconds", 0.0) for x in results)
     setup_pre_s = r_setup.stats.get("precond_seconds", 0.0)
     pinfo = ctx.ildl_device_info() if args.precond == "ildl" else None
@@ -434,8 +454,12 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f)
+    tsrc = (f"profiles/traffic_{args.workload}.json: rocprofv3 --pmc FETCH_SIZE and "
+            f"WRITE_SIZE of the same kernel on this workload, calibrated with a copy kernel "
+            f"(stored figure, not measured in this run)")
     roof_spmv = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get("spmv"),
+                 "traffic_source": tsrc if traffic.get("spmv") else None,
                  "kernel": "k_spmv_sell (symmetric SpMV, sliced ELL)" if r.stats.get(
                      "spmv_lanes") == 0 else "k_spmv (symmetric SpMV, CSR)",
                  "alg_bytes": alg, "avg_ms": spmv_ms}
@@ -456,6 +480,8 @@ def main():
                     "frac": sw_ach / HBM_PEAK_GBS,
                     "traffic": (traffic.get("sweep")
                                 if traffic.get("sweep_kernel") == kname else None),
+                    "traffic_source": (tsrc if traffic.get("sweep_kernel") == kname
+                                       and traffic.get("sweep") else None),
                     "kernel": f"{kname} (ILDL level-1 Jacobi sweep, {kdesc})",
                     "alg_bytes": sw_bytes, "avg_ms": sw_ms}
 
@@ -495,6 +521,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
+        native = O.use_native()  # -O3 -march=native, built on this host
         fac_s = 0.0
         if args.precond == "diag":
             mv = O.matvec_upper(ia, ja, a)
@@ -530,6 +557,8 @@ def main():
         per_mv = cpu_s / max(ro.iter, 1)
         est_solve = per_mv * r.iter
         cpu = {"value": r.neig / est_solve, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
+               "build": "gcc -O3 -march=native" if native else "gcc -O3",
+               "host_cpu": _cpu_model(), "host_nproc": os.cpu_count(),
                "end_to_end_value": r.neig / (est_solve + fac_s),
                "sample": (f"oracle (C restatement of the reference drivers and multilevel "
                           f"apply, 1 thread) on the first {ro.iter} MATVECs of the same solve "
@@ -570,6 +599,7 @@ def main():
                    "host_syncs": r.stats.get("host_syncs"),
                    "spmv_lanes": r.stats.get("spmv_lanes"),
                    "max_rel_err_vs_closed_form": max_rel_err,
+                   "failed_timed_solves": failed_solves,
                    "max_res": float(np.max(r.res)) if r.neig else None,
                    # relative residual ||A x - theta x|| / (||A||_1 ||x||), ||x|| = 1
                    "max_rel_res": (float(np.max(r.res)) / norm1(ia, ja, a)) if r.neig else None,

@@ -129,9 +129,10 @@ typedef struct {
 } jdhip_batch_item;
 /* Solve every item on HIP device `device` with nworkers worker threads, each
  * owning a context and a HIP stream; items are taken largest first.  Each
- * item starts from the state of a fresh process, so its results equal those
- * of a single jdhip_pjd call.  No reference counterpart (the reference solves
- * one problem per call).  Returns the number of items with INFO != 0, or -1
+ * item starts from the state of a fresh process and makes its set-up choices
+ * from its inputs alone (options ildl_autotune = 0, ildl_tune_time = 0), so
+ * its results equal those of a single jdhip_pjd call with these options.  No
+ * reference counterpart (the reference solves one problem per call).  Returns the number of items with INFO != 0, or -1
  * on invalid arguments. */
 int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers);
 
@@ -164,7 +165,11 @@ int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
  * (work vectors of the apply: 1 FP32, 0 FP64, -1 FP32 for definite factors
  * only, the default), "inner_fp32" (1 = FP32 inner solver; default 0: FP64
  * inner solver as in the reference), "inner_spmv64" (FP32 inner solver: 1 =
- * its SpMV reads the FP64 values, the default).  0 on success, -2 for an
+ * its SpMV reads the FP64 values, the default), "ildl_tune_time" (1 = sweep
+ * counts are kept only when timed faster than exact solves, the default; 0 =
+ * whenever accurate), "fail_setup" (k > 0: the k-th preconditioner set-up of
+ * a call reports failure, to exercise the INFO = -54321 path of
+ * JDaux.F:431-441; default 0).  0 on success, -2 for an
  * unknown name, -3 for an invalid value. */
 int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value);
 
@@ -243,6 +248,10 @@ typedef struct {
     int precond_levels;     /* sparse levels + dense level (ILDL) */
     int precond_negpiv;     /* negative pivots before |D| conversion */
     int precond_sweeps;     /* triangular solves: 0 exact, k Jacobi sweeps */
+    /* per set-up of the last call (as printed by the reference, JDaux.F:1226-1232):
+     * used DROPTOL and fill factor (nnzU + 2n) / nnz(A_upper) */
+    double setup_droptols[64];
+    double setup_fills[64];
 } jdhip_stats;
 void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out);
 /* last error message of ctx ("" if none) */

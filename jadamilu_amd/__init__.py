@@ -33,7 +33,8 @@ class JdhipStats(C.Structure):
                 ("n_precond_setups", C.c_int), ("spmv_lanes", C.c_int),
                 ("setup_shifts", C.c_double * 64), ("precond_seconds", C.c_double),
                 ("fill", C.c_double), ("precond_levels", C.c_int),
-                ("precond_negpiv", C.c_int), ("precond_sweeps", C.c_int)]
+                ("precond_negpiv", C.c_int), ("precond_sweeps", C.c_int),
+                ("setup_droptols", C.c_double * 64), ("setup_fills", C.c_double * 64)]
 
 
 class JdhipBatchItem(C.Structure):

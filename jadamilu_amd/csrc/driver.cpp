@@ -30,6 +30,12 @@ namespace jdhip {
 
 struct Solver;
 
+// A re-factorisation requested inside a preconditioner solve failed
+// (JDaux.F:431-441): the run ends with INFO = -54321, IJOB = 0.
+struct PrecondSetupFailed : std::runtime_error {
+    PrecondSetupFailed() : std::runtime_error("preconditioner re-setup failed") {}
+};
+
 struct SolverOps : Ops {
     Solver *S = nullptr;
     bool mv = false, pc = false;
@@ -72,7 +78,11 @@ struct Solver {
     int ildl_vec32 = -1;   // FP32 factor: work vectors FP32 (1), FP64 (0), by definiteness (-1)
     bool ildl_autotune = true;  // kernel form of each product by timing at set-up
     double ildl_tune_tol = 0.0;  // sweep tuning tolerance (0: by definiteness)
+    bool ildl_tune_time = true;  // sweep counts also checked against exact solves by timing
     int ildl_form = -1;  // >= 0: one kernel form for every product of the factor
+    // set-up number (1-based, counted from the start of a call) that is made
+    // to fail, so that callers can exercise the INFO = -54321 path; 0: none
+    int fail_setup = 0;
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -97,7 +107,7 @@ struct Solver {
     double sigma = 0;
     double *a = nullptr, *eigs = nullptr, *shift = nullptr, *droptol = nullptr, *mem = nullptr;
     int *ja = nullptr, *ia = nullptr, *neig = nullptr, *icntl = nullptr;
-    std::vector<double> shifts_log;
+    std::vector<double> shifts_log, droptols_log, fills_log;
     int nsetups = 0;
     double precond_seconds = 0.0;
 
@@ -399,6 +409,8 @@ void Solver::pjdinit(int isearch_, double sigma_, double shift_in) {
         dglprecsetup(isearch_, shift_in, &s0, &rdiag);
         shift0 = s0;
         shifts_log.push_back(shift0);
+        droptols_log.push_back(0.0);  // no DROPTOL or fill for the diagonal
+        fills_log.push_back(0.0);
         ++nsetups;
         return;
     }
@@ -474,6 +486,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         opt.threads = ildl_threads;
         IldlHost h;
         ildl_factor_host(n, ia, ja, a, shift0, opt, h);
+        if (fail_setup > 0 && nsetups + 1 == fail_setup) h.res.ierr = -1;
         const IldlResult &r = h.res;
         memused = r.fill;
         double totaltime =
@@ -521,6 +534,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         ildl->vec32 = ildl_vec32;
         ildl->autotune = ildl_autotune;
         ildl->tune_tol = ildl_tune_tol;
+        ildl->tune_time = ildl_tune_time;
         ildl->force_form = ildl_form;
         ildl->attach(h, ildl_sweeps);
         ildl_info = ildl->result;
@@ -532,6 +546,8 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         droptol0 = *droptol;
         condest0 = r.condest;
         shifts_log.push_back(shift0);
+        droptols_log.push_back(*droptol);
+        fills_log.push_back(r.fill);
         ++nsetups;
         return;
     }
@@ -654,7 +670,7 @@ void Solver::adapt() {
     }
     if (reinit) {
         pjdinit(std::max(isearch, 1), sigma, *shift);
-        if (ierr != 0) throw std::runtime_error("preconditioner setup failed");
+        if (ierr != 0) throw PrecondSetupFailed();
         tic_ = std::chrono::steady_clock::now();
         if (factspd0 < 0 && factspd <= 0) factspd = factspd0;
     }
@@ -682,6 +698,8 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
     if (*ijob == 0) {
         ierr = 0;
         shifts_log.clear();
+        droptols_log.clear();
+        fills_log.clear();
         nsetups = 0;
         precond_seconds = 0.0;
         if ((icntl[1] != -1 && icntl[1] != -2) || factdgl == 0) {
@@ -728,8 +746,20 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
         ops.f32 = internal_mv && flagsingle && inner_fp32;
         eng.set_ops(&ops);
     }
-    eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
-             ndx1, ndx2, iprint, info, gap);
+    try {
+        eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol,
+                 ijob, ndx1, ndx2, iprint, info, gap);
+    } catch (const PrecondSetupFailed &) {
+        // JDaux.F:431-441 ends the run here.  The reference marks the
+        // diagonal preconditioner as current (factdgl = 1) without building
+        // it; this context drops the failed factor instead, so that a later
+        // call with ICNTL(2) = -1/-2 builds a new preconditioner.
+        release_precond();
+        factdgl = 0;
+        *info = -54321;
+        *ijob = 0;
+        return;
+    }
     if (*ijob == 3) *ijob = 1;
     if (*ijob == 0 && ierr == 0) suggest(*neig, eigs, *gap);
 }
@@ -816,6 +846,10 @@ void jdhip_get_stats(jdhip_ctx *ctx, jdhip_stats *out) {
     out->spmv_lanes = S.A.lpr;
     for (size_t i = 0; i < S.shifts_log.size() && i < 64; ++i)
         out->setup_shifts[i] = S.shifts_log[i];
+    for (size_t i = 0; i < S.droptols_log.size() && i < 64; ++i) {
+        out->setup_droptols[i] = S.droptols_log[i];
+        out->setup_fills[i] = S.fills_log[i];
+    }
     out->precond_seconds = S.precond_seconds;
     out->fill = S.ildl_info.fill;
     out->precond_levels = S.ildl_info.nlev;
@@ -969,6 +1003,8 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_vec32") S.ildl_vec32 = (int)value;
     else if (k == "ildl_autotune") S.ildl_autotune = value != 0.0;
     else if (k == "ildl_tune_tol") S.ildl_tune_tol = value;
+    else if (k == "fail_setup") S.fail_setup = (int)value;
+    else if (k == "ildl_tune_time") S.ildl_tune_time = value != 0.0;
     else if (k == "ildl_form") {
         const int f = (int)value;
         if (f != -1 && f != 0 && f != 1 && f != 4 && f != 8 && f != 16 && f != 32 && f != 64)
@@ -1147,6 +1183,7 @@ int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps) {
         S.ildl->vec32 = S.ildl_vec32;
         S.ildl->autotune = S.ildl_autotune;
         S.ildl->tune_tol = S.ildl_tune_tol;
+        S.ildl->tune_time = S.ildl_tune_time;
         S.ildl->force_form = S.ildl_form;
         S.ildl->attach(f->h, sweeps);
         S.factdgl = -1;

@@ -565,7 +565,7 @@ class IldlImpl : public IldlImplBase {
         clear_graphs();
     }
 
-    int tune_sweeps(double tol) override {
+    int tune_sweeps(double tol, bool by_time) override {
         set_all(0);
         if (lv_.empty()) return 0;
         tuning_ = true;  // per-level counts change between applies
@@ -636,7 +636,10 @@ class IldlImpl : public IldlImplBase {
             sweeps_ = 0;
             for (auto &D : lv_) sweeps_ = std::max(sweeps_, D.k);
             // 3) keep the sweeps only if they beat the exact solve
-            if (timed(2) >= t_exact) set_all(0);
+            if (by_time && timed(2) >= t_exact) set_all(0);
+        } else {
+            // no uniform count reproduces the exact apply: keep exact solves
+            set_all(0);
         }
         tuning_ = false;
         clear_graphs();
@@ -974,7 +977,8 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     // the 2nd-6th eigenvalues nearest sigma instead of the 1st-5th.)
     if (force_form >= 0) impl_->set_forms(force_form);
     else if (autotune) impl_->autotune_forms();
-    if (sweeps < 0) impl_->tune_sweeps(tune_tol > 0.0 ? tune_tol : (h.spd ? 0.03 : 1e-4));
+    if (sweeps < 0)
+        impl_->tune_sweeps(tune_tol > 0.0 ? tune_tol : (h.spd ? 0.03 : 1e-4), tune_time);
     result.sweeps = impl_->sweeps();
     impl_->set_graphs(graphs);
 }

@@ -51,7 +51,9 @@ class IldlImplBase {
     // Pick the triangular-solve mode by measurement: the fastest of the exact
     // level-scheduled solve and k Jacobi sweeps whose apply stays within
     // relative distance tol of the exact apply on a fixed random vector.
-    virtual int tune_sweeps(double tol) = 0;
+    // by_time = false skips the timing comparison: sweeps are kept whenever
+    // they are accurate (a choice that depends on the inputs only).
+    virtual int tune_sweeps(double tol, bool by_time) = 0;
     // Time reps launches of one Jacobi sweep over the first level's L_B (the
     // largest single kernel of the apply) on the context stream.
     virtual int time_sweep(int reps, double *avg_ms, long *alg_bytes) = 0;
@@ -107,6 +109,7 @@ class IldlPrecond {
     bool autotune = true;  // kernel forms by measurement (else by row length)
     int force_form = -1;   // >= 0: this form for every product (tests, experiments)
     double tune_tol = 0.0;  // sweep tuning tolerance (0: 3e-2 definite, 1e-4 indefinite)
+    bool tune_time = true;  // sweeps kept only if faster than exact solves (timed)
 
   private:
     DevCtx *ctx_;

@@ -197,6 +197,26 @@ def cleanup():
     lib().dpjdcleanup_()
 
 
+def legacy_stats():
+    """Statistics (Context.stats() fields) of the last legacy-symbol call."""
+    from . import JdhipStats
+    L = lib()
+    s = JdhipStats()
+    L.jdhip_get_stats(L.jdhip_legacy_context(), C.byref(s))
+    k = s.n_precond_setups
+    return {"n_precond_setups": k, "setup_shifts": list(s.setup_shifts[:k]),
+            "setup_droptols": list(s.setup_droptols[:k]), "setup_fills": list(s.setup_fills[:k]),
+            "inner_iters": s.inner_iters, "precond_sweeps": s.precond_sweeps}
+
+
+def legacy_set_option(name, value):
+    """jdhip_set_option on the context behind the legacy symbols."""
+    L = lib()
+    rc = L.jdhip_set_option(L.jdhip_legacy_context(), name.encode(), float(value))
+    if rc != 0:
+        raise ValueError(f"option {name}: {rc}")
+
+
 def reset_legacy():
     """Return the process-global context behind the legacy symbols to the
     state of a fresh process (preconditioner released, saved COMMON state
@@ -236,6 +256,8 @@ class Context:
         return {"inner_iters": s.inner_iters, "host_syncs": s.host_syncs,
                 "n_precond_setups": s.n_precond_setups, "spmv_lanes": s.spmv_lanes,
                 "setup_shifts": list(s.setup_shifts[: s.n_precond_setups]),
+                "setup_droptols": list(s.setup_droptols[: s.n_precond_setups]),
+                "setup_fills": list(s.setup_fills[: s.n_precond_setups]),
                 "precond_seconds": s.precond_seconds, "fill": s.fill,
                 "precond_levels": s.precond_levels, "precond_negpiv": s.precond_negpiv,
                 "precond_sweeps": s.precond_sweeps}

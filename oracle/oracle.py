@@ -15,6 +15,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libjdoracle.so")
+_NATIVE_PATH = os.path.join(_HERE, "_build", "libjdoracle_native.so")
 _lib = None
 
 _i = C.POINTER(C.c_int)
@@ -24,6 +25,21 @@ _d = C.POINTER(C.c_double)
 def build() -> str:
     subprocess.check_call(["make", "-s", "-C", _HERE])
     return _LIB_PATH
+
+
+def use_native() -> bool:
+    """Build and load the -O3 -march=native copy of the oracle (for timing on
+    the machine it runs on).  Must be called before the first lib() call;
+    returns False (and keeps the portable -O3 build) if it does not build."""
+    global _LIB_PATH
+    if _lib is not None:
+        return _LIB_PATH == _NATIVE_PATH
+    try:
+        subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "native"])
+    except (OSError, subprocess.CalledProcessError):
+        return False
+    _LIB_PATH = _NATIVE_PATH
+    return True
 
 
 def lib():

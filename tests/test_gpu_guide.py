@@ -1,0 +1,150 @@
+"""The user guide's printed DPJD / DPJDREVCOM runs, end to end on the GPU.
+
+Each run calls the legacy drop-in symbols (dpjd_ / dpjdrevcom_, host arrays)
+with the parameters of the guide's sample, and checks what the guide prints
+(tests/golden/userguide_runs.json, extracted from Doc/userguide.ps):
+eigenvalues to 1e-10 relative, the MATVEC count within 3, and for every
+preconditioner set-up the used SHIFT and DROPTOL (the adaptive
+re-factorisation sequence of PJDRVCOM, JDaux.F:282-449) and the fill factor.
+
+The triangular solves run exactly (ildl_sweeps = 0), as the reference's do:
+the tuned Jacobi sweeps change the preconditioner slightly and with it the
+MATVEC counts.
+"""
+import numpy as np
+import pytest
+
+from jadamilu_amd import matrices as M
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+N = 1000
+# set-ups whose fill the host factor does not reproduce (same list as the
+# CPU twin tests/test_guide_setups.py)
+FILL_OPEN = {"EXAMPLE1", "LISTING5_ADAPTIVE", "LISTING5_STATIC"}
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _fresh(sweeps=0):
+    from jadamilu_amd import jd
+    jd.reset_legacy()
+    jd.legacy_set_option("ildl_sweeps", sweeps)
+    jd.legacy_set_option("fail_setup", 0)
+
+
+def _check(name, g, r, st, fill_checked=True):
+    print(name, "matvecs", r.iter, "guide", g["matvec"], "setups", st["n_precond_setups"],
+          "shifts", st["setup_shifts"], "droptols", st["setup_droptols"], "fills",
+          [round(f, 4) for f in st["setup_fills"]])
+    assert r.info == 0 and r.neig == 5, (r.info, r.neig)
+    np.testing.assert_allclose(np.sort(r.eigs), np.sort([t["eig"] for t in g["table"]]),
+                               rtol=1e-10, atol=1e-13)
+    assert (r.res <= 2e-10).all(), r.res
+    assert abs(r.iter - g["matvec"]) <= 3, (r.iter, g["matvec"])
+    gs = g["precond_setups"]
+    assert st["n_precond_setups"] == len(gs)
+    for k, s in enumerate(gs):
+        assert abs(st["setup_shifts"][k] - s["shift"]) <= 1e-4 * max(1.0, abs(s["shift"]))
+        assert abs(st["setup_droptols"][k] - s["droptol"]) <= 1e-12 + 1e-6 * s["droptol"]
+        if fill_checked:
+            assert abs(st["setup_fills"][k] - s["fill"]) <= 0.005 + 1e-12
+
+
+@pytest.mark.parametrize("name,isearch", [("EXAMPLE1", 2), ("EXAMPLE2", 0)])
+def test_dpjd_guide(golden, name, isearch):
+    """samples/EXAMPLE1.f (5 eigenvalues nearest 0, one set-up) and
+    samples/EXAMPLE2.f (5 smallest, ILU recomputed twice with smaller DROPTOL)."""
+    from jadamilu_amd import jd
+    _dev()
+    g = golden[name]["calls"][0]
+    ia, ja, a = M.tridiag(N, np.arange(N) * 1.0, 5.0)
+    _fresh()
+    r = jd.pjd_host(N, a, ja, ia, neig=5, isearch=isearch, sigma=0.0, madspace=20,
+                    maxiter=1000, tol=1e-10, droptol=1e-3, mem=20.0)
+    _check(name, g, r, jd.legacy_stats(), fill_checked=name not in FILL_OPEN)
+    if "suggested_droptol" in g:
+        assert "%.2E" % r.droptol == g["suggested_droptol"]
+    if "suggested_shift" in g:
+        assert "%.2E" % r.shift == g["suggested_shift"]
+
+
+@pytest.mark.parametrize("name,icntl2", [("LISTING5_ADAPTIVE", 0), ("LISTING5_STATIC", 1)])
+def test_dpjdrevcom_listing5(golden, name, icntl2):
+    """Listing 5/6 of the guide: DPJDREVCOM with the ILU of tridiag(5,0,5)
+    while the caller's matvec applies tridiag(5,0,5) + 0.1 * ones ones^T."""
+    from jadamilu_amd import jd
+    _dev()
+    g = golden[name]["calls"][0]
+    ia, ja, a = M.tridiag(N, np.zeros(N), 5.0)
+
+    def mv(x, y):
+        y[:] = 5.0 * np.concatenate([[0.0], x[:-1]]) + 5.0 * np.concatenate([x[1:], [0.0]])
+        y += 0.1 * x.sum()
+
+    _fresh()
+    r = jd.pjdrevcom_host(N, a, ja, ia, mv, neig=5, isearch=0, sigma=0.0, madspace=20,
+                          maxiter=1000, tol=1e-10, droptol=1e-3, mem=20.0,
+                          icntl=(0, icntl2, 0, 0, 0))
+    _check(name, g, r, jd.legacy_stats(), fill_checked=name not in FILL_OPEN)
+
+
+def test_dpjd_example11_two_stages(golden):
+    """samples/EXAMPLE11.f: TOL 1e-3 with an exact (DROPTOL 0) factor, then
+    re-entry with ICNTL(2) = -1 (factor kept), the first call's eigenvectors
+    as NINIT = NEIG initial guesses, ISEARCH = 1 and TOL 1e-10."""
+    from jadamilu_amd import jd
+    _dev()
+    g1, g2 = golden["EXAMPLE11"]["calls"]
+    ia, ja, a = M.tridiag(N, np.arange(1, N + 1) * 1.0, 5.0)
+    _fresh()
+    r1 = jd.pjd_host(N, a, ja, ia, neig=5, isearch=0, sigma=0.0, madspace=20, maxiter=1000,
+                     tol=1e-3, droptol=0.0, mem=20.0)
+    st1 = jd.legacy_stats()
+    print("EXAMPLE11 stage 1 matvecs", r1.iter, "guide", g1["matvec"], st1["setup_shifts"],
+          st1["setup_droptols"], st1["setup_fills"])
+    assert r1.info == 0 and r1.neig == 5
+    np.testing.assert_allclose(np.sort(r1.eigs), np.sort([t["eig"] for t in g1["table"]]),
+                               rtol=1e-6)
+    assert abs(r1.iter - g1["matvec"]) <= 3, (r1.iter, g1["matvec"])
+    assert st1["n_precond_setups"] == 1
+    assert st1["setup_shifts"][0] == g1["precond_setups"][0]["shift"]
+    assert abs(st1["setup_fills"][0] - g1["precond_setups"][0]["fill"]) <= 0.005
+    r2 = jd.pjd_host(N, a, ja, ia, neig=5, isearch=1, sigma=0.0, ninit=5, madspace=20,
+                     maxiter=1000, tol=1e-10, shift=r1.shift, droptol=r1.droptol, mem=20.0,
+                     icntl=(0, -1, 0, 0, 0), x0=r1.x, eigs0=r1.eigs)
+    st2 = jd.legacy_stats()
+    print("EXAMPLE11 stage 2 matvecs", r2.iter, "guide", g2["matvec"], "setups",
+          st2["n_precond_setups"])
+    assert r2.info == 0 and r2.neig == 5
+    np.testing.assert_allclose(np.sort(r2.eigs), np.sort([t["eig"] for t in g2["table"]]),
+                               rtol=1e-10)
+    assert abs(r2.iter - g2["matvec"]) <= 3, (r2.iter, g2["matvec"])
+    assert st2["n_precond_setups"] == 0  # the factor of the first call is reused
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_failed_setup_returns_54321(which):
+    """A preconditioner set-up that fails ends the run with INFO = -54321 and
+    IJOB = 0 (JDaux.F:431-441 for a re-factorisation inside the run, PJD.F
+    for the first one); the context stays usable.  The failure is forced from
+    the host (option fail_setup)."""
+    from jadamilu_amd import jd
+    _dev()
+    ia, ja, a = M.tridiag(N, np.arange(N) * 1.0, 5.0)
+    _fresh()
+    jd.legacy_set_option("fail_setup", which)
+    r = jd.pjd_host(N, a, ja, ia, neig=5, isearch=0, sigma=0.0, madspace=20, maxiter=1000,
+                    tol=1e-10, droptol=1e-3, mem=20.0)
+    st = jd.legacy_stats()
+    print("fail_setup", which, "info", r.info, "setups done", st["n_precond_setups"])
+    assert r.info == -54321
+    assert st["n_precond_setups"] == which - 1
+    _fresh()
+    r = jd.pjd_host(N, a, ja, ia, neig=5, isearch=0, sigma=0.0, madspace=20, maxiter=1000,
+                    tol=1e-10, droptol=1e-3, mem=20.0)
+    assert r.info == 0 and r.neig == 5

@@ -180,23 +180,6 @@ def test_tuned_sweeps_stay_close_to_exact(gen, spd):
     ctx.close()
 
 
-@pytest.mark.parametrize("name,isearch", [("EXAMPLE1", 2), ("EXAMPLE2", 0)])
-def test_dpjd_guide(golden, name, isearch):
-    from jadamilu_amd import jd
-    _dev()
-    g = golden[name]["calls"][0]
-    n = 1000
-    ia, ja, a = M.tridiag(n, np.arange(n) * 1.0, 5.0)
-    jd.reset_legacy()
-    r = jd.pjd_host(n, a, ja, ia, neig=5, isearch=isearch, sigma=0.0, madspace=20,
-                    maxiter=1000, tol=1e-10, droptol=1e-3, mem=20.0)
-    print(name, "matvecs", r.iter, "guide", g["matvec"], "eigs", r.eigs)
-    assert r.info == 0 and r.neig == 5
-    np.testing.assert_allclose(np.sort(r.eigs), np.sort([t["eig"] for t in g["table"]]),
-                               rtol=1e-10, atol=1e-13)
-    assert (r.res <= 2e-10).all()
-
-
 def test_dpjd_laplace3d_closed_form():
     from jadamilu_amd import jd
     _dev()
@@ -258,4 +241,22 @@ def test_graph_replays_match_eager_apply(sweeps, single):
         assert np.array_equal(zd.cpu().numpy(), z_eager), rep
         other.mul_(1.0001)  # eager work between replays
         torch.cuda.synchronize()
+    ctx.close()
+
+
+def test_tuning_without_accurate_sweeps_keeps_exact_solves():
+    """When no uniform sweep count reproduces the exact apply (tolerance
+    1e-300), the tuned mode keeps exact triangular solves on every level."""
+    from jadamilu_amd.ildl import IldlFactor
+    from jadamilu_amd.jd import Context
+    _dev()
+    ia, ja, a = M.laplace((20, 19, 18))
+    n = ia.size - 1
+    f = IldlFactor(n, ia, ja, a, droptol=1e-3, spd=True)
+    ctx = Context(0)
+    ctx.set_option("ildl_tune_tol", 1e-300)
+    ctx.attach_ildl(f, sweeps=-1)
+    assert ctx.stats()["precond_sweeps"] == 0
+    lv = ctx.ildl_level_sweeps()
+    assert len(lv) >= 1 and all(k == 0 for k in lv), lv
     ctx.close()

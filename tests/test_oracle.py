@@ -134,3 +134,29 @@ def test_sortrx():
         idx = O.sortrx(d) - 1
         assert np.all(np.diff(d[idx]) >= 0)
         assert sorted(idx) == list(range(n))
+
+
+@pytest.mark.parametrize("name,isearch,shift,droptol,tol", [
+    ("EXAMPLE1", 2, 0.0, 1e-3, 1e-10),
+    ("EXAMPLE11", 0, -8.0, 0.0, 1e-3),
+])
+def test_guide_ildl_runs(golden, name, isearch, shift, droptol, tol):
+    """jdo_pjd (the oracle's DPJD) preconditioned by the host multilevel
+    factor at the guide's printed set-up (one set-up in these runs, so the
+    oracle's missing re-factorisation logic does not enter): eigenvalues to
+    1e-10 of the printed ones and the MATVEC count within 3 of the guide's."""
+    from jadamilu_amd.ildl import IldlFactor
+    g = golden[name]["calls"][0]
+    n = 1000
+    diag = np.arange(n) * 1.0 if name == "EXAMPLE1" else np.arange(1, n + 1) * 1.0
+    ia, ja, a = M.tridiag(n, diag, 5.0)
+    f = IldlFactor(n, ia, ja, a, shift=shift, droptol=droptol, condest=5.0, spd=isearch <= 1)
+    assert f.stats()["ierr"] == 0
+    r = O.pjd_ildl(n, ia, ja, a, f.export(), neig=5, isearch=isearch, sigma=0.0, madspace=20,
+                   maxiter=1000, tol=tol, icntl=(0, 1, 0, 0, 0))
+    print(name, "oracle matvecs", r.iter, "guide", g["matvec"])
+    assert r.info == 0 and r.neig == 5
+    rt = 1e-10 if tol <= 1e-10 else 1e-6
+    np.testing.assert_allclose(np.sort(r.eigs), np.sort([t["eig"] for t in g["table"]]),
+                               rtol=rt, atol=rt)
+    assert abs(r.iter - g["matvec"]) <= 3, (r.iter, g["matvec"])
