@@ -147,6 +147,25 @@ int jdhip_set_matrix(jdhip_ctx *ctx, int n, const int *ia, const int *ja, const 
 /* y = A x on ctx's stream, x and y DEVICE arrays of n doubles. */
 int jdhip_spmv(jdhip_ctx *ctx, const double *x_dev, double *y_dev);
 
+/* Y = A X for k vectors (SpMM, MATVECA of JDaux.F:3397-3418 on a block): x_dev
+ * and y_dev DEVICE arrays, column-major with leading dimensions ldx, ldy >= n
+ * (the operator set by jdhip_set_matrix).  Every matrix entry is read once
+ * per 8 vectors.  0 on success, -1 bad arguments, -2 no matrix / HIP error. */
+int jdhip_spmm(jdhip_ctx *ctx, int k, const double *x_dev, long ldx, double *y_dev, long ldy);
+
+/* ---- dense contractions on the FP64 matrix cores ------------------------- */
+/* C = V^T W (k x m, k, m <= 64) for tall V (n x k) and W (n x m), DEVICE,
+ * column-major: the Gram products of the Rayleigh-Ritz steps (JDaux.F:
+ * 1651-1660, 2013-2031, 2165-2200), computed with v_mfma_f64_16x16x4_f64 and
+ * summed deterministically; on ctx's stream, with ctx's 4 MiB of partial-sum
+ * scratch.  0 on success, -1 bad arguments, -2 HIP error. */
+int jdhip_gram(jdhip_ctx *ctx, long n, int k, int m, const double *v_dev, long ldv,
+               const double *w_dev, long ldw, double *c_dev, int ldc);
+/* Y = V Q (n x j) for V (n x k), Q (k x j), k, j <= 64, DEVICE, column-major,
+ * on the matrix cores: the basis update of SHRINK (JDaux.F:2418-2462). */
+int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, long ldv,
+                    const double *q_dev, int ldq, double *y_dev, long ldy);
+
 /* ---- options ------------------------------------------------------------- */
 /* Build options of ctx (no reference counterpart; defaults reproduce the
  * reference's behaviour, except "ildl_sweeps"): "ildl_sweeps" (0 = exact

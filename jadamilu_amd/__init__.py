@@ -75,6 +75,11 @@ SIGNATURES = {
     "jdhip_stream": (_v, [_v]),
     "jdhip_set_matrix": (C.c_int, [_v, C.c_int, _i, _i, _d, C.c_int]),
     "jdhip_spmv": (C.c_int, [_v, _v, _v]),
+    "jdhip_spmm": (C.c_int, [_v, C.c_int, _v, C.c_long, _v, C.c_long]),
+    "jdhip_gram": (C.c_int, [_v, C.c_long, C.c_int, C.c_int, _v, C.c_long, _v, C.c_long, _v,
+                             C.c_int]),
+    "jdhip_tall_gemm": (C.c_int, [_v, C.c_long, C.c_int, C.c_int, _v, C.c_long, _v, C.c_int,
+                                  _v, C.c_long]),
     "jdhip_get_stats": (None, [_v, C.POINTER(JdhipStats)]),
     "jdhip_set_option": (C.c_int, [_v, C.c_char_p, C.c_double]),
     "jdhip_precond_apply": (C.c_int, [_v, _v, _v]),

@@ -1019,6 +1019,57 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     return 0;
 }
 
+int jdhip_spmm(jdhip_ctx *ctx, int k, const double *x_dev, long ldx, double *y_dev, long ldy) {
+    try {
+        if (!ctx) return -1;
+        Solver &S = *ctx->S;
+        if (!S.A.rowptr) throw std::runtime_error("jdhip_spmm: no matrix set");
+        if (k < 0 || (k > 0 && (!x_dev || !y_dev || ldx < S.A.n || ldy < S.A.n))) return -1;
+        if (k == 0) return 0;
+        JDHIP_CHECK(hipSetDevice(S.ctx.device));
+        jdhip::spmm(S.ctx.stream, S.A, k, x_dev, ldx, y_dev, ldy);
+        JDHIP_CHECK(hipGetLastError());
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -2;
+    }
+}
+
+int jdhip_gram(jdhip_ctx *ctx, long n, int k, int m, const double *v_dev, long ldv,
+               const double *w_dev, long ldw, double *c_dev, int ldc) {
+    try {
+        if (!ctx || n < 0 || k < 0 || m < 0 || k > 64 || m > 64) return -1;
+        if (k == 0 || m == 0) return 0;
+        if (!c_dev || ldc < k || (n > 0 && (!v_dev || !w_dev || ldv < n || ldw < n))) return -1;
+        Solver &S = *ctx->S;
+        JDHIP_CHECK(hipSetDevice(S.ctx.device));
+        jdhip::gram(S.ctx.stream, n, k, m, v_dev, ldv, w_dev, ldw, c_dev, ldc, S.ctx.d_gram);
+        JDHIP_CHECK(hipGetLastError());
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -2;
+    }
+}
+
+int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, long ldv,
+                    const double *q_dev, int ldq, double *y_dev, long ldy) {
+    try {
+        if (!ctx || n < 0 || k <= 0 || j < 0 || k > 64 || j > 64) return -1;
+        if (n == 0 || j == 0) return 0;
+        if (!v_dev || !q_dev || !y_dev || ldv < n || ldy < n || ldq < k) return -1;
+        Solver &S = *ctx->S;
+        JDHIP_CHECK(hipSetDevice(S.ctx.device));
+        jdhip::tall_gemm_mfma(S.ctx.stream, v_dev, ldv, k, q_dev, ldq, j, y_dev, ldy, n);
+        JDHIP_CHECK(hipGetLastError());
+        return 0;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        return -2;
+    }
+}
+
 int jdhip_precond_apply(jdhip_ctx *ctx, double *z_dev, const double *r_dev) {
     try {
         Solver &S = *ctx->S;

@@ -37,6 +37,8 @@ void DevCtx::init(int dev) {
     JDHIP_CHECK(hipHostMalloc(&h_pin, sizeof(double) * 8192, hipHostMallocDefault));
     JDHIP_CHECK(hipHostMalloc(&h_sc, sizeof(InnerScalars), hipHostMallocDefault));
     std::memset(h_sc, 0, sizeof(InnerScalars));
+    JDHIP_CHECK(hipMalloc(&d_gram, sizeof(double) * kGramScratch));
+    JDHIP_CHECK(hipMalloc(&d_gout, sizeof(double) * kMaxCols * kMaxCols));
 }
 
 void DevCtx::release() {
@@ -48,6 +50,8 @@ void DevCtx::release() {
     hipFree(d_small);
     hipFree(d_sc);
     hipFree(d_tmp);
+    hipFree(d_gram);
+    hipFree(d_gout);
     hipHostFree(h_pin);
     hipHostFree(h_sc);
     hipStreamDestroy(stream);
@@ -272,10 +276,10 @@ int JdEngine::shrink(int n, int j, int k, int ucol_off, int iserch) {
     put(dq, q.data(), (long)k * j);
     ctx_->ensure_tmp((long)n * j);
     hipStream_t s = ctx_->stream;
-    tall_gemm(s, X_ + vv, n, k, dq, k, j, ctx_->d_tmp, n, n);
+    tall_gemm_mfma(s, X_ + vv, n, k, dq, k, j, ctx_->d_tmp, n, n);
     JDHIP_CHECK(hipMemcpyAsync(X_ + vv, ctx_->d_tmp, sizeof(double) * n * j,
                                hipMemcpyDeviceToDevice, s));
-    tall_gemm(s, X_ + ww, n, k, dq, k, j, ctx_->d_tmp, n, n);
+    tall_gemm_mfma(s, X_ + ww, n, k, dq, k, j, ctx_->d_tmp, n, n);
     JDHIP_CHECK(hipMemcpyAsync(X_ + ww, ctx_->d_tmp, sizeof(double) * n * j,
                                hipMemcpyDeviceToDevice, s));
     const int ld = modspace;
@@ -296,29 +300,33 @@ int JdEngine::shrink(int n, int j, int k, int ucol_off, int iserch) {
 void JdEngine::gram_initial(int n) {
     hipStream_t s = ctx_->stream;
     const int ld = modspace;
-    double a[kMaxCols], b[kMaxCols];
+    std::vector<double> C((size_t)J * J);
     if (iserch == 2) {
         // W := W - sigma V over the J initial columns (contiguous)
         // (sigma folded into axpyalpha by the caller)
         axpy(s, X_ + ww, axpyalpha, X_ + bv, (long)n * J);
-        for (int c = 0; c < J; ++c) {
-            multidot(s, ctx_->rb, X_ + ww, n, c + 1, X_ + ww + (long)c * n, n, ctx_->slot(0));
-            fetch(a, ctx_->slot(0), c + 1);
-            for (int r = 0; r <= c; ++r) NN[r + c * ld] = a[r];
-        }
+        // NN = W^T W (upper triangle), one MFMA Gram product
+        gram(s, n, J, J, X_ + ww, n, X_ + ww, n, ctx_->d_gout, J, ctx_->d_gram);
+        fetch(C.data(), ctx_->d_gout, (long)J * J);
+        for (int c = 0; c < J; ++c)
+            for (int r = 0; r <= c; ++r) NN[r + c * ld] = C[r + (size_t)c * J];
     }
-    for (int c = 0; c < J; ++c) {
-        multidot(s, ctx_->rb, X_ + vv, n, c + 1, X_ + ww + (long)c * n, n, ctx_->slot(0));
-        multidot(s, ctx_->rb, X_ + ww, n, c + 1, X_ + vv + (long)c * n, n, ctx_->slot(1));
-        double buf[2 * kMaxCols];
-        fetch(buf, ctx_->slot(0), 2 * kMaxCols);
-        for (int r = 0; r <= c; ++r) {
-            a[r] = buf[r];
-            b[r] = buf[kMaxCols + r];
-            MM[r + c * ld] = a[r] + b[r];
-        }
-    }
+    // MM = (V^T W + W^T V) / 2 from one Gram product C = V^T W
+    gram_sym(n, X_ + vv, X_ + ww, J, C.data());
     for (auto &v : MM) v *= 0.5;
+}
+
+// MM(r, c) = V(:,r).W(:,c) + W(:,r).V(:,c) for r <= c < J (upper triangle),
+// from one MFMA Gram product C = V^T W (the column-by-column GEMV 'C'
+// updates of JDaux.F:1651-1660 and 2165-2200 in one launch).
+void JdEngine::gram_sym(int n, const double *V, const double *W, int J, double *C) {
+    hipStream_t s = ctx_->stream;
+    const int ld = modspace;
+    gram(s, n, J, J, V, n, W, n, ctx_->d_gout, J, ctx_->d_gram);
+    fetch(C, ctx_->d_gout, (long)J * J);
+    for (int c = 0; c < J; ++c)
+        for (int r = 0; r <= c; ++r)
+            MM[r + c * ld] = C[r + (size_t)c * J] + C[c + (size_t)r * J];
 }
 
 // STOPTEST (JDaux.F:2542-2672), standard problem; xn2 = ||x||^2 (device).
@@ -1009,14 +1017,8 @@ L225:
     ++*iter;
     if (IJ < J) goto L220;
     {
-        const int ld = modspace;
-        for (int c = 0; c < J; ++c) {
-            double buf[2 * kMaxCols];
-            multidot(s, rb, D(vv), n, c + 1, D(ww + (long)c * n), n, ctx_->slot(0));
-            multidot(s, rb, D(ww), n, c + 1, D(vv + (long)c * n), n, ctx_->slot(1));
-            fetch(buf, ctx_->slot(0), 2 * kMaxCols);
-            for (int r = 0; r <= c; ++r) MM[r + c * ld] = buf[r] + buf[kMaxCols + r];
-        }
+        std::vector<double> C((size_t)J * J);
+        gram_sym(n, D(vv), D(ww), J, C.data());
         for (auto &v : MM) v *= 0.5;
     }
     IJ = solprojep(J, MM.data(), NP.data(), &sclr1, &sclr2, REIGW, 3);
@@ -1028,10 +1030,10 @@ L225:
         double *dq = ctx_->d_small + 4096;
         put(dq, q.data(), (long)J * J);
         ctx_->ensure_tmp((long)n * J);
-        tall_gemm(s, D(vv), n, J, dq, J, J, ctx_->d_tmp, n, n);
+        tall_gemm_mfma(s, D(vv), n, J, dq, J, J, ctx_->d_tmp, n, n);
         JDHIP_CHECK(hipMemcpyAsync(D((long)n * (I - 1) + xx), ctx_->d_tmp, nb * J,
                                    hipMemcpyDeviceToDevice, s));
-        tall_gemm(s, D(ww), n, J, dq, J, J, ctx_->d_tmp, n, n);
+        tall_gemm_mfma(s, D(ww), n, J, dq, J, J, ctx_->d_tmp, n, n);
         JDHIP_CHECK(hipMemcpyAsync(D(vv), ctx_->d_tmp, nb * J, hipMemcpyDeviceToDevice, s));
     }
     nrc = 0.0;

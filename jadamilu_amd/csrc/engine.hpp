@@ -42,6 +42,8 @@ struct DevCtx {
     InnerScalars *h_sc = nullptr;
     double *d_tmp = nullptr;     // n x ncols scratch (tall products)
     long tmp_len = 0;
+    double *d_gram = nullptr;    // partial sums of the MFMA Gram products (kGramScratch)
+    double *d_gout = nullptr;    // Gram result, up to kMaxCols x kMaxCols
 
     void init(int dev);
     void release();
@@ -103,6 +105,7 @@ class JdEngine {
                   std::vector<double> &w, int is);
     int shrink(int n, int j, int k, int ucol_off, int iserch);
     void gram_initial(int n);
+    void gram_sym(int n, const double *V, const double *W, int J, double *C);
     void fetch(double *host, const double *dev, long cnt);
     void put(double *dev, const double *host, long cnt);
     void stoptest(int n, double tol, double *rk, double alprho, double gk, double *theta,

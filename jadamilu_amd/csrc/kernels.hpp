@@ -7,6 +7,7 @@
 // round trip.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 namespace jdhip {
@@ -57,6 +58,21 @@ void dot(hipStream_t s, const RedBuf &rb, const double *x, const double *y, long
 // out[j*ldo+i] = sum_k V[k*ldv+i] Q[k+j*ldq]   (n x K times K x J)
 void tall_gemm(hipStream_t s, const double *V, long ldv, int K, const double *Q,
                int ldq, int J, double *out, long ldo, long n);
+// ---- FP64 matrix-core (MFMA) contractions, kernels_mfma.hip ----
+// Partial sums of gram() hold at most this many doubles (4 MiB).
+constexpr long kGramScratch = 524288;
+// Workgroups of gram() for these sizes: at most 512, at most one per 64 rows,
+// and nblk * K * M <= kGramScratch.
+int gram_blocks(long n, int K, int M);
+// C(i + j*ldc) = sum_r V[i*ldv + r] W[j*ldw + r]  (C = V^T W, K, M <= 64),
+// with gram_blocks(n, K, M) * K * M doubles of partial sums in scratch
+// (device).  Deterministic (partials summed in block order).  0 on success,
+// -1 if K or M exceeds 64.
+int gram(hipStream_t s, long n, int K, int M, const double *V, long ldv, const double *W,
+         long ldw, double *C, int ldc, double *scratch);
+// Y = V Q on the matrix cores (K, J <= 64); same result layout as tall_gemm.
+int tall_gemm_mfma(hipStream_t s, const double *V, long ldv, int K, const double *Q, int ldq,
+                   int J, double *Y, long ldy, long n);
 // out = {index of max |x_i| (first), x_idx} as doubles
 void argmax_abs(hipStream_t s, const RedBuf &rb, const double *x, long n, double *out);
 
@@ -89,6 +105,11 @@ struct CsrDev {
 template <typename T>
 void spmv(hipStream_t s, const RedBuf &rb, const CsrDev &A, const T *x, T *y, double eta,
           const T *u, double *udot_out);
+
+// Y = A X for k vectors (column-major, leading dimensions ldx, ldy), FP64,
+// in passes of up to 8 vectors over the matrix (kernels_spmm.hip)
+void spmm(hipStream_t s, const CsrDev &A, int k, const double *x, long ldx, double *y,
+          long ldy);
 
 // ---- preconditioners / conversion ----
 template <typename T>

@@ -320,6 +320,26 @@ class Context:
         if rc != 0:
             raise RuntimeError(self.error())
 
+    def spmm(self, k, x_ptr, ldx, y_ptr, ldy):
+        """Y = A X for k device vectors (column-major)."""
+        rc = self._L.jdhip_spmm(self.h, k, C.c_void_p(x_ptr), ldx, C.c_void_p(y_ptr), ldy)
+        if rc != 0:
+            raise RuntimeError(self.error() or f"jdhip_spmm: {rc}")
+
+    def gram(self, n, k, m, v_ptr, ldv, w_ptr, ldw, c_ptr, ldc):
+        """C = V^T W on the FP64 matrix cores (device arrays, column-major)."""
+        rc = self._L.jdhip_gram(self.h, n, k, m, C.c_void_p(v_ptr), ldv, C.c_void_p(w_ptr), ldw,
+                                C.c_void_p(c_ptr), ldc)
+        if rc != 0:
+            raise RuntimeError(self.error() or f"jdhip_gram: {rc}")
+
+    def tall_gemm(self, n, k, j, v_ptr, ldv, q_ptr, ldq, y_ptr, ldy):
+        """Y = V Q on the FP64 matrix cores (device arrays, column-major)."""
+        rc = self._L.jdhip_tall_gemm(self.h, n, k, j, C.c_void_p(v_ptr), ldv, C.c_void_p(q_ptr),
+                                     ldq, C.c_void_p(y_ptr), ldy)
+        if rc != 0:
+            raise RuntimeError(self.error() or f"jdhip_tall_gemm: {rc}")
+
     def spmv(self, x_ptr, y_ptr):
         rc = self._L.jdhip_spmv(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr))
         if rc != 0:
