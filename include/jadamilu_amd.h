@@ -176,7 +176,9 @@ int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, l
  * "ildl_autotune" (1 = kernel form of each factor product chosen by timing,
  * the default), "ildl_form" (-1 = automatic; 0, 1, 4, 8, 16, 32, 64 = that form for
  * every product), "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee,
- * 2 approximate minimum degree, the default), "ildl_dense_max" (largest final
+ * 2 approximate minimum degree, the default), "ildl_coupling" (0 = F blocks and
+ * two solves with B per level, as the reference, the default; 1 = L_CB blocks,
+ * see jdhip_ildl_coupling), "ildl_dense_max" (largest final
  * level factored densely), "spmv_lanes", "ildl_graph" (1: replay the
  * preconditioner apply from captured HIP graphs; default 0), "ildl_storage",
  * "ildl_xcd", "ildl_sigma" (device layout of the factor), "ildl_threads"
@@ -207,13 +209,21 @@ int jdhip_precond_apply(jdhip_ctx *ctx, double *z_dev, const double *r_dev);
 typedef struct jdhip_ildl jdhip_ildl;
 jdhip_ildl *jdhip_ildl_factor(int n, const int *ia, const int *ja, const double *a, double shift,
                               double droptol, double condest, int spd, int single, int ordering,
-                              int dense_max, int *ierr);
+                              int dense_max, int coupling, int *ierr);
 void jdhip_ildl_free(jdhip_ildl *f);
+/* Coupling block each level stores (the lc_* arrays of jdhip_ildl_level_data):
+ * 0 = F^T = A_CB of the level's scaled, permuted input, applied with two
+ * solves with B per level as DSYMAMGsol does (symAMGsol.c:110-631, F from
+ * symAMGextract.c:112), the default; 1 = L_CB, the factor's rows of the
+ * deferred pivots, applied as the block LDL^T substitution (one solve with
+ * L_B and one with L_B^T per level). */
+int jdhip_ildl_coupling(jdhip_ildl *f);
 /* number of sparse levels; *dense_m = order of the dense final level */
 int jdhip_ildl_nlevels(jdhip_ildl *f, int *dense_m);
 int jdhip_ildl_level_sizes(jdhip_ildl *f, int lev, int *n, int *nB, long *nnz_lb, long *nnz_lc);
 /* copy out level lev: perm[n] (position -> level input index), scal[n],
- * strict lower L_B (CSR, nB rows), L_CB (CSR, n-nB rows), pivots d[nB] */
+ * strict lower L_B (CSR, nB rows), coupling block (CSR, n-nB rows: F^T or
+ * L_CB, see jdhip_ildl_coupling), pivots d[nB] */
 int jdhip_ildl_level_data(jdhip_ildl *f, int lev, int *perm, double *scal, int *lb_ptr,
                           int *lb_idx, double *lb_val, int *lc_ptr, int *lc_idx, double *lc_val,
                           double *d);

@@ -84,7 +84,8 @@ SIGNATURES = {
     "jdhip_set_option": (C.c_int, [_v, C.c_char_p, C.c_double]),
     "jdhip_precond_apply": (C.c_int, [_v, _v, _v]),
     "jdhip_ildl_factor": (_v, [C.c_int, _i, _i, _d, C.c_double, C.c_double, C.c_double, C.c_int,
-                               C.c_int, C.c_int, C.c_int, _i]),
+                               C.c_int, C.c_int, C.c_int, C.c_int, _i]),
+    "jdhip_ildl_coupling": (C.c_int, [_v]),
     "jdhip_ildl_free": (None, [_v]),
     "jdhip_ildl_nlevels": (C.c_int, [_v, _i]),
     "jdhip_ildl_level_sizes": (C.c_int, [_v, C.c_int, _i, _i, _l, _l]),

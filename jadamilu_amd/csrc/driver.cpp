@@ -63,6 +63,7 @@ struct Solver {
     int factdgl = 0, factspd = 0, prvdr = 0;
     // build options of the multilevel ILDL
     int ildl_sweeps = -1, ildl_ordering = 2, ildl_dense_max = 1500;
+    int ildl_coupling = 0;  // 0: F blocks (the reference's apply), 1: L_CB blocks
     // device layout of the factor: original-index storage order, XCD-contiguous
     // block order, SELL sorting window 256 (the fastest apply measured on the
     // 128^3 Laplacian: 1.47 ms against 1.84 ms for factor order / window 1024)
@@ -482,6 +483,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         opt.single = flagsingle;
         opt.sweeps = ildl_sweeps;
         opt.ordering = ildl_ordering;
+        opt.coupling = ildl_coupling;
         opt.dense_max = ildl_dense_max;
         opt.threads = ildl_threads;
         IldlHost h;
@@ -992,6 +994,10 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     std::string k(name);
     if (k == "ildl_sweeps") S.ildl_sweeps = (int)value;
     else if (k == "ildl_ordering") S.ildl_ordering = (int)value;
+    else if (k == "ildl_coupling") {
+        if (value != 0.0 && value != 1.0) return -3;
+        S.ildl_coupling = (int)value;
+    }
     else if (k == "ildl_dense_max") S.ildl_dense_max = (int)value;
     else if (k == "spmv_lanes") S.A.lpr = (int)value;
     else if (k == "inner_fp32") S.inner_fp32 = value != 0.0;
@@ -1151,7 +1157,7 @@ extern "C" {
 
 jdhip_ildl *jdhip_ildl_factor(int n, const int *ia, const int *ja, const double *a, double shift,
                               double droptol, double condest, int spd, int single, int ordering,
-                              int dense_max, int *ierr) {
+                              int dense_max, int coupling, int *ierr) {
     auto *p = new jdhip_ildl();
     try {
         jdhip::IldlOptions opt;
@@ -1161,6 +1167,7 @@ jdhip_ildl *jdhip_ildl_factor(int n, const int *ia, const int *ja, const double 
         opt.single = single != 0;
         opt.ordering = ordering;
         opt.dense_max = dense_max > 0 ? dense_max : 1500;
+        opt.coupling = coupling ? 1 : 0;
         jdhip::ildl_factor_host(n, ia, ja, a, shift, opt, p->h);
         *ierr = p->h.res.ierr;
     } catch (const std::exception &e) {
@@ -1171,6 +1178,8 @@ jdhip_ildl *jdhip_ildl_factor(int n, const int *ia, const int *ja, const double 
 }
 
 void jdhip_ildl_free(jdhip_ildl *f) { delete f; }
+
+int jdhip_ildl_coupling(jdhip_ildl *f) { return f->h.single ? f->h.ff.coupling : f->h.fd.coupling; }
 
 int jdhip_ildl_nlevels(jdhip_ildl *f, int *dense_m) {
     if (f->h.single) {

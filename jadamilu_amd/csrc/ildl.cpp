@@ -397,7 +397,9 @@ class IldlImpl : public IldlImplBase {
         JDHIP_CHECK(hipDeviceSynchronize());
     }
 
-    void tri_solve(const DevLevel<T, W> &D, bool fwd) {
+    // Triangular solve with L_B (fwd) or L_B^T in place on x (nB entries).
+    void tri_solve(const DevLevel<T, W> &D, bool fwd) { tri_solve(D, fwd, D.buf); }
+    void tri_solve(const DevLevel<T, W> &D, bool fwd, W *x) {
         hipStream_t s = ctx_->stream;
         const int *ptr = fwd ? D.lb_ptr : D.lt_ptr;
         const int *idx = fwd ? D.lb_idx : D.lt_idx;
@@ -407,25 +409,24 @@ class IldlImpl : public IldlImplBase {
             const std::vector<int> &lp = fwd ? D.fwd_ptr : D.bwd_ptr;
             const int *rows = fwd ? D.fwd_rows : D.bwd_rows;
             for (size_t l = 1; l + 1 < lp.size(); ++l)  // level 0 rows have no entries
-                ildl_trsv_level<T, W>(s, rows + lp[l], lp[l + 1] - lp[l], ptr, idx, val, D.buf);
+                ildl_trsv_level<T, W>(s, rows + lp[l], lp[l + 1] - lp[l], ptr, idx, val, x);
             return;
         }
-        // Jacobi sweeps x <- b - N x from x = b (b = D.buf).  The first sweep
-        // reads b itself; the last writes into D.buf, which is safe because
-        // every row reads only its own b entry and all x entries come from
+        // Jacobi sweeps y <- b - N y from y = b (b = x).  The first sweep
+        // reads b itself; the last writes into x, which is safe because
+        // every row reads only its own b entry and all y entries come from
         // the other buffer (needs at least 2 sweeps; 1 sweep goes via t1).
         const int lanes = fwd ? D.wf : D.wb;
-        const W *xin = D.buf;
+        const W *xin = x;
         W *bufs[2] = {D.t1, D.t2};
         for (int k = 0; k < nsw; ++k) {
             const bool last = (k == nsw - 1) && nsw >= 2;
-            W *xout = last ? D.buf : bufs[k & 1];
-            sweep_launch(D, fwd, lanes, D.buf, xin, xout);
+            W *xout = last ? x : bufs[k & 1];
+            sweep_launch(D, fwd, lanes, x, xin, xout);
             xin = xout;
         }
         if (nsw == 1)
-            JDHIP_CHECK(
-                hipMemcpyAsync(D.buf, D.t1, sizeof(W) * D.nB, hipMemcpyDeviceToDevice, s));
+            JDHIP_CHECK(hipMemcpyAsync(x, D.t1, sizeof(W) * D.nB, hipMemcpyDeviceToDevice, s));
     }
 
     // One Jacobi sweep xout = b - N xin of the level's L_B (fwd) or L_B^T,
@@ -718,8 +719,15 @@ class IldlImpl : public IldlImplBase {
                 ildl_gather<W, W>(s, D.n, D.perm, D.scal, P.buf + P.nB, D.buf);
             }
             tri_solve(D, true);
-            if (D.nC > 0) sub_launch(D, false, D.wc, D.buf, D.buf + D.nB);
-            ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
+            if (fform()) {
+                // y_B = B^-1 b_B, then b_C -= F^T y_B
+                ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
+                tri_solve(D, false);
+                if (D.nC > 0) sub_launch(D, false, D.wc, D.buf, D.buf + D.nB);
+            } else {
+                if (D.nC > 0) sub_launch(D, false, D.wc, D.buf, D.buf + D.nB);
+                ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
+            }
         }
         const W *zc = nullptr;
         if (f_.m > 0) {
@@ -729,14 +737,28 @@ class IldlImpl : public IldlImplBase {
         }
         for (int l = L - 1; l >= 0; --l) {
             DevLevel<T, W> &D = lv_[l];
-            if (D.nC > 0) sub_launch(D, true, D.wct, zc, D.buf);
-            tri_solve(D, false);
+            if (fform()) {
+                if (D.nC > 0) {
+                    // x_B = y_B - B^-1 F x_C; D.out is free until the scatter
+                    W *t = D.out;
+                    ildl_zero<W>(s, D.nB, t);
+                    sub_launch(D, true, D.wct, zc, t);
+                    tri_solve(D, true, t);
+                    ildl_dscale<W>(s, D.nB, D.dinv, t);
+                    tri_solve(D, false, t);
+                    ildl_add<W>(s, D.nB, t, D.buf);
+                }
+            } else {
+                if (D.nC > 0) sub_launch(D, true, D.wct, zc, D.buf);
+                tri_solve(D, false);
+            }
             if (l == 0) ildl_scatter<W, TV>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, z);
             else ildl_scatter<W, W>(s, D.n, D.nB, D.perm, D.scal, D.buf, zc, D.out);
             zc = D.out;
         }
     }
 
+    bool fform() const { return f_.coupling == 0; }
     int nlevels() const override { return (int)f_.levels.size(); }
     int dense_m() const override { return f_.m; }
     void level_sizes(int lev, int *n, int *nB, long *nnz_lb, long *nnz_lc) const override {
@@ -778,8 +800,11 @@ class IldlImpl : public IldlImplBase {
         long b = 0;
         for (const auto &D : lv_) {
             const long k = D.k > 0 ? D.k : 1;
-            b += k * (mat_bytes(D.sf, D.wf, D.nnz_lb) + mat_bytes(D.sb, D.wb, D.nnz_lb)) +
-                 2 * k * (4L * D.nB + 3 * v * D.nB);
+            // F blocks: a level with a coarse part solves with B twice
+            const long solves = (fform() && D.nC > 0) ? 2 : 1;
+            b += solves * (k * (mat_bytes(D.sf, D.wf, D.nnz_lb) + mat_bytes(D.sb, D.wb, D.nnz_lb)) +
+                           2 * k * (4L * D.nB + 3 * v * D.nB));
+            if (solves == 2) b += 2 * v * D.nB * 3;  // second D^-1, zero, add
             b += mat_bytes(D.sc, D.wc, D.nnz_lc) + mat_bytes(D.sct, D.wct, D.nnz_lc) +
                  2 * (4L * (D.nC + D.nB + 2) + v * (D.nB + D.nC));
             b += 2 * (4L + v) * D.n + 2 * v * D.n + 2 * v * D.nB;
@@ -862,6 +887,7 @@ class IldlImpl : public IldlImplBase {
 // Copy of a double factor with every stored value rounded to float.
 static IldlMultilevel<float> round_to_float(const IldlMultilevel<double> &d) {
     IldlMultilevel<float> f;
+    f.coupling = d.coupling;
     for (const auto &L : d.levels) {
         IldlLevel<float> F;
         F.n = L.n;
@@ -899,6 +925,7 @@ void ildl_factor_host(int n, const int *ia, const int *ja, const double *a, doub
     prm.spd = opt.spd;
     prm.ordering = opt.ordering;
     prm.dense_max = opt.dense_max;
+    prm.coupling = opt.coupling;
     prm.threads = opt.threads > 0
                       ? opt.threads
                       : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
@@ -919,13 +946,31 @@ void ildl_factor_host(int n, const int *ia, const int *ja, const double *a, doub
     res.nnz = st.nnz;
     res.nneg = st.nneg;
     res.condest = st.condest_used;
-    // fill-in factor as the reference prints it: (nnz(U) + 2n) / nnz(A)
-    // (symilupacknnz.c; JDaux.F:1221-1228), nnz(U) = stored off-diagonal
-    // factor entries (sparse levels) plus the packed dense level
+    // fill-in factor as the reference prints it (JDaux.F:1221-1228): the
+    // count of symilupacknnz.c:80-125 over nnzU = the strict triangular
+    // factor entries and the coupling block entries of every sparse level;
+    // a dense final level of order j is weighted as sparse storage of its
+    // indices and values would be, nnzU - j(j+1)/2 + 2n - j + j(j+1)/3
+    // (j(j-1)/2 of its entries counted first); divided by nnz(A_upper)
     {
-        long offd = st.nnz;
-        for (int v : st.level_nB) offd -= v;
-        res.fill = (double)(offd + 2L * n) / (double)std::max(nnzu, 1L);
+        long nnz_lev = 0;
+        int m = 0;
+        auto count = [&](const auto &ml) {
+            for (const auto &L : ml.levels)
+                nnz_lev += (long)L.lb_idx.size() + (long)L.lc_idx.size();
+            m = ml.m;
+        };
+        if (opt.single) count(out.ff);
+        else count(out.fd);
+        double tot;
+        if (m > 0) {
+            const long j = m;
+            const long a1 = nnz_lev + (j * (j - 1)) / 2 - (j * (j + 1)) / 2 + 2L * n - j;
+            tot = std::floor((double)a1 + (double)(j * (j + 1)) / 3.0);
+        } else {
+            tot = (double)(nnz_lev + 2L * n);
+        }
+        res.fill = tot / (double)std::max(nnzu, 1L);
     }
     res.level_n = st.level_n;
     res.level_nB = st.level_nB;

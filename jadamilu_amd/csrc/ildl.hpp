@@ -19,6 +19,7 @@ struct IldlOptions {
     int ordering = 2;  // 0 natural, 1 RCM, 2 AMD
     int dense_max = 1500;
     int threads = 0;     // host threads of the factorisation (0: all cores, at most 16)
+    int coupling = 0;    // 0: F blocks, two B solves per level (the reference's); 1: L_CB
 };
 
 struct IldlResult {

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstdint>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -162,7 +163,13 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     // ---- Crout loop ----
     std::vector<int8_t> status(n, 0);  // 0 pending, 1 accepted, 2 deferred
     std::vector<int> bpos(n, -1), cpos(n, -1);
-    std::vector<T> d(n, T(0)), est(n, T(0));
+    std::vector<T> d(n, T(0));
+    // inverse-norm estimators of sympiluc.F: est1 (the 1-norm estimate of
+    // the columns of L^-1, w(n+1:2n)), est2 (the improved estimate with its
+    // own right-hand side, w(2n+1:3n)), peaks (counted essential entries per
+    // column, jw(ne+1:ne+n))
+    std::vector<double> est1(n, 0.0), est2(n, 0.0);
+    std::vector<int> peaks(n, 0);
     // column arena (accepted columns, rows sorted)
     std::vector<long> cbeg(n, 0), cend(n, 0), cur(n, 0);
     std::vector<int> crow;
@@ -227,29 +234,89 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
                 if (drow[t] != k) acc(drow[t], -coef * dval[t], k);
         }
         bool skip = !(wk != T(0)) || !std::isfinite((double)wk);
-        T x = T(0);
+        const double dk = (double)wk;
+        // --- first estimate (sympiluc.F:1465-1510): right-hand side +-1 in
+        // component k, the sign giving the larger 1-norm of the updated
+        // column estimates; skip k if that estimate exceeds condest
+        double x1 = 0.0;
         if (!skip) {
-            T xp = T(1) - est[k], xm = T(-1) - est[k];
+            const double xp = (1.0 - est1[k]) / dk, xm = (-1.0 - est1[k]) / dk;
             double mup = 0, nup = 0, mum = 0, num = 0;
             for (int i : wlist) {
-                T li = wv[i] / wk;
-                double a1 = std::fabs((double)(est[i] + xp * li));
-                double a2 = std::fabs((double)(est[i] + xm * li));
+                const double u = (double)wv[i];
+                const double a1 = std::fabs(est1[i] + xp * u), a2 = std::fabs(est1[i] + xm * u);
                 nup = std::max(nup, a1);
                 mup += a1;
                 num = std::max(num, a2);
                 mum += a2;
             }
-            double nn;
+            double axp;
             if (mup >= mum) {
-                x = xp;
-                nn = nup;
+                x1 = xp;
+                axp = std::fabs(1.0 - est1[k]);
             } else {
-                x = xm;
-                nn = num;
+                x1 = xm;
+                axp = std::fabs(-1.0 - est1[k]);
+                nup = num;
             }
-            if (std::max(std::fabs((double)x), nn) > condest) skip = true;
+            if (std::max(nup, axp) > condest) skip = true;
         }
+        // peaks and the update of the first estimate over the whole row
+        // (before dropping), sympiluc.F:1611-1640
+        {
+            int m = peaks[k];
+            if (std::fabs(est1[k]) <= (double)m) ++m;
+            peaks[k] = m;
+            if (!skip) {
+                for (int i : wlist) {
+                    const double y = x1 * (double)wv[i];
+                    if (std::fabs(y) * peaks[i] >= std::fabs(est1[i]) * m) peaks[i] = m;
+                    est1[i] += y;
+                }
+                est1[k] = std::max(std::fabs(1.0 - est1[k]), std::fabs(-1.0 - est1[k]));
+            } else {
+                est1[k] = 1.0;
+            }
+        }
+        // --- improved estimate (sympiluc.F:1700-1860): the sign that
+        // seriously increases more entries than it decreases
+        if (!skip) {
+            const double xp = (1.0 - est2[k]) / dk, xm = (-1.0 - est2[k]) / dk;
+            int jp = 0, jm = 0;
+            double nup = 0, num = 0;
+            for (int i : wlist) {
+                const double u = (double)wv[i];
+                const double ax = std::fabs(est2[i]);
+                double ay = std::fabs(est2[i] + xp * u);
+                nup = std::max(nup, ay);
+                if (ay > 2.0 * ax && ay > 0.5) ++jp;
+                if (2.0 * ay < ax && ax > 0.5) --jp;
+                ay = std::fabs(est2[i] + xm * u);
+                num = std::max(num, ay);
+                if (ay > 2.0 * ax && ay > 0.5) ++jm;
+                if (2.0 * ay < ax && ax > 0.5) --jm;
+            }
+            double x2, axp;
+            if (jp >= jm) {
+                x2 = xp;
+                axp = std::fabs(1.0 - est2[k]);
+            } else {
+                x2 = xm;
+                axp = std::fabs(-1.0 - est2[k]);
+                nup = num;
+            }
+            if (std::max(nup, axp) > condest) skip = true;
+            if (!skip) {
+                for (int i : wlist) est2[i] += x2 * (double)wv[i];
+                est1[k] = std::max({est1[k], std::fabs(1.0 - est2[k]), std::fabs(-1.0 - est2[k])});
+            } else {
+                est1[k] = 1.0;
+                for (int i : wlist) est1[i] -= x1 * (double)wv[i];  // undo the first update
+            }
+        }
+        // infinity-norm estimate from the 1-norm and the peaks (sympiluc.F:2033)
+        if (peaks[k] > 2) est1[k] = 2.0 * est1[k] / peaks[k];
+        if (est1[k] < 1.0) est1[k] = 1.0;
         if (skip) {
             status[k] = 2;
             cpos[k] = nC++;
@@ -258,14 +325,16 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         status[k] = 1;
         bpos[k] = nB++;
         d[k] = wk;
-        maxcond = std::max(maxcond, std::fabs((double)x));
+        // dropping (sympiluc.F:2191-2239): U(k,l) with |U(k,l)| max_m |L(m,k)|
+        // <= droptol / maxcondest, maxcondest the largest estimate so far
+        maxcond = std::max(maxcond, est1[k]);
         double lmax = 0.0;
         for (int i : wlist) lmax = std::max(lmax, std::fabs((double)(wv[i] / wk)));
         const double thr = prm.droptol / maxcond;
         kept.clear();
         for (int i : wlist) {
             T li = wv[i] / wk;
-            if (std::fabs((double)(wk * li)) * lmax > thr) kept.emplace_back(i, li);
+            if (std::fabs((double)wv[i]) * lmax > thr) kept.emplace_back(i, li);
         }
         std::sort(kept.begin(), kept.end(),
                   [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
@@ -273,7 +342,6 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         for (auto &kv : kept) {
             crow.push_back(kv.first);
             cval.push_back(kv.second);
-            est[kv.first] += x * kv.second;
             // append to row list of kv.first (keeps column order increasing)
             rnext.push_back(-1);
             rcol.push_back(k);
@@ -285,8 +353,11 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         }
         cend[k] = (long)crow.size();
         cur[k] = cbeg[k];
+        // entries produced so far (L_B and L_CB rows): a runaway guard at
+        // twice the elbow space; the elbow limit itself applies to the
+        // stored factor once the level is complete
         nnz_stored += (long)kept.size() + 1;
-        if (nnz_limit > 0 && nnz_stored > nnz_limit) {
+        if (nnz_limit > 0 && nnz_stored > 2 * nnz_limit) {
             out.ierr = -3;
             return;
         }
@@ -480,6 +551,32 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         }
         S = std::move(U);
     }
+    if (prm.coupling == 0) {
+        // the level's coupling block as stored and applied by the reference:
+        // F^T = A_CB of the scaled, permuted input (symAMGextract.c:112);
+        // L_CB was needed only for the Schur complement above
+        std::vector<int> fp(nC + 1, 0), fi;
+        std::vector<T> fv;
+        std::vector<std::pair<int, T>> row;
+        for (int c = 0; c < nC; ++c) {
+            const int k = cnode[c];
+            row.clear();
+            for (int t = ap[k]; t < ap[k + 1]; ++t) {
+                const int j = ai[t];
+                if (status[j] == 1 && av[t] != T(0)) row.emplace_back(bpos[j], av[t]);
+            }
+            std::sort(row.begin(), row.end(),
+                      [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
+            for (auto &e : row) {
+                fi.push_back(e.first);
+                fv.push_back(e.second);
+            }
+            fp[c + 1] = (int)fi.size();
+        }
+        L.lc_ptr.swap(fp);
+        L.lc_idx.swap(fi);
+        L.lc_val.swap(fv);
+    }
     if (verbose())
         std::fprintf(stderr,
                      "jdhip ildl: level n=%d nB=%d nC=%d condest=%g  ordering %.2fs  "
@@ -647,34 +744,61 @@ IldlStats ildl_factor(const SymCsr &A0, const IldlParams &prm, IldlMultilevel<T>
                       long nnz_upper_ref) {
     IldlStats st;
     out = IldlMultilevel<T>();
+    out.coupling = prm.coupling;
     long nnz_stored = 0;
     long limit = (prm.mem > 0.0) ? (long)(prm.mem * (double)nnz_upper_ref) : 0;
     SymCsr cur = A0;
     double condest = prm.condest;
     st.condest_used = condest;
     int nneg = 0;
+    const double condest0 = prm.condest;
     for (int lev = 0; lev < prm.max_levels && cur.n > 0; ++lev) {
-        // dense switch (symAMGsetup.c:450): the level is dense enough that a
-        // packed factor (n(n+1)/2) costs less than 3 x its sparse storage
+        // dense switch of the coarser levels (symAMGsetup.c:450): 3 nnz(S)
+        // > n(n+1)/2, nnz(S) counting the stored upper triangle with the
+        // diagonal; this build also caps the dense order at dense_max
         const double full = 0.5 * (double)cur.n * (cur.n + 1.0);
-        if (cur.n <= 64 ||
-            (cur.n <= prm.dense_max && 3.0 * (double)cur.idx.size() >= full))
-            break;
+        const double nnz_up = 0.5 * (double)(cur.idx.size() + (size_t)cur.n);
+        if (lev > 0 && cur.n <= prm.dense_max && 3.0 * nnz_up > full) break;
         LevelOut<T> lo;
-        double ce = condest;
+        // condest doubling while more than 75 % of the pivots are deferred,
+        // up to 16 x the level's starting value and 1024 x the initial one;
+        // the value reached carries over to the next level
+        // (symAMGsetup.c:777-809)
+        const double condest1 = condest;
         long saved = nnz_stored;
-        for (int tries = 0;; ++tries) {
+        for (;;) {
             nnz_stored = saved;
             lo = LevelOut<T>();
-            factor_level<T>(cur, prm, ce, nnz_stored, limit, lo);
+            factor_level<T>(cur, prm, condest, nnz_stored, limit, lo);
             if (lo.ierr) {
                 st.ierr = lo.ierr;
                 return st;
             }
-            if (lo.nC <= 0.75 * cur.n || tries >= 4) break;
-            ce *= 2.0;
+            if (!(lo.nC > 0.75 * cur.n && condest < 1024.0 * condest0 &&
+                  condest < 16.0 * condest1))
+                break;
+            condest *= 2.0;
         }
-        st.condest_used = std::max(st.condest_used, ce);
+        if (lo.nC > 0.75 * cur.n) {
+            // no sensible reduction: the level is factored completely,
+            // every pivot accepted (the ILDLC fallback, symAMGsetup.c:1315)
+            nnz_stored = saved;
+            lo = LevelOut<T>();
+            factor_level<T>(cur, prm, std::numeric_limits<double>::infinity(), nnz_stored,
+                            limit, lo);
+            if (lo.ierr) {
+                st.ierr = lo.ierr;
+                return st;
+            }
+        }
+        st.condest_used = std::max(st.condest_used, condest);
+        // elbow space (MEM): the stored factor of the levels so far (L_B,
+        // coupling block, pivots) must fit into MEM x nnz(A_upper)
+        nnz_stored = saved + (long)lo.L.lb_idx.size() + (long)lo.L.lc_idx.size() + lo.L.nB;
+        if (limit > 0 && nnz_stored > limit) {
+            st.ierr = -3;
+            return st;
+        }
         st.level_n.push_back(cur.n);
         st.level_nB.push_back(lo.L.nB);
         for (auto v : lo.L.d) nneg += (v < T(0));

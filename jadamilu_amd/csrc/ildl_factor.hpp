@@ -34,7 +34,10 @@ struct IldlLevel {
     // strict lower L_B (rows/cols = B positions), CSR
     std::vector<int> lb_ptr, lb_idx;
     std::vector<T> lb_val;
-    // L_CB (rows = C positions 0..nC-1, cols = B positions), CSR
+    // coupling block (rows = C positions 0..nC-1, cols = B positions), CSR:
+    // F^T, the rows of the level's scaled, permuted input matrix between C
+    // and B (coupling 0, the reference's structure), or L_CB, the factor's
+    // own rows of the deferred pivots (coupling 1)
     std::vector<int> lc_ptr, lc_idx;
     std::vector<T> lc_val;
     std::vector<T> d;  // pivots d_k of B (SPD-converted: |d_k|)
@@ -43,6 +46,14 @@ struct IldlLevel {
 template <typename T>
 struct IldlMultilevel {
     std::vector<IldlLevel<T>> levels;
+    // 0: every level stores F^T = A_CB and the apply solves with B twice
+    //    (forward and backward block substitution of DSYMAMGsol,
+    //    src/ilupack/symAMGsol.c:110-631; the F block is extracted by
+    //    DSYMAMGextract, symAMGextract.c:112);
+    // 1: every level stores L_CB and the apply is the block LDL^T
+    //    substitution [L_B 0; L_CB I] [D 0; 0 S] [L_B^T L_CB^T; 0 I]
+    //    (one solve with L_B and one with L_B^T per level)
+    int coupling = 0;
     int m = 0;                 // size of the dense final level (0: none)
     std::vector<double> dinv;  // m x m inverse of the (converted) final block
     long nnz_factor = 0;       // stored entries (L, D, dense)
@@ -57,6 +68,7 @@ struct IldlParams {
     int max_levels = 40;
     bool spd = false;        // convert pivots to |d| (ISEARCH <= 1)
     int ordering = 2;        // 0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum degree
+    int coupling = 0;        // coupling block stored per level (IldlMultilevel::coupling)
     int threads = 1;         // host threads for the Schur complements
 };
 

@@ -281,6 +281,17 @@ __global__ __launch_bounds__(kB) void k_dscale(int n, const double *__restrict__
         x[i] = (W)((double)x[i] * dinv[i]);
 }
 
+template <typename W>
+__global__ __launch_bounds__(kB) void k_zero(int n, W *x) {
+    for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
+        x[i] = W(0);
+}
+template <typename W>
+__global__ __launch_bounds__(kB) void k_add(int n, const W *__restrict__ t, W *__restrict__ y) {
+    for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
+        y[i] = (W)((double)y[i] + (double)t[i]);
+}
+
 // one wave per row of the dense inverse
 template <typename T, typename W>
 __global__ __launch_bounds__(kB) void k_dense_gemv(int m, const T *__restrict__ Minv,
@@ -399,6 +410,20 @@ void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x) {
     if (n <= 0) return;
     k_dscale<W><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
 }
+template <typename W>
+void ildl_zero(hipStream_t s, int n, W *x) {
+    if (n <= 0) return;
+    k_zero<W><<<blocks_for(n, kB), kB, 0, s>>>(n, x);
+}
+template <typename W>
+void ildl_add(hipStream_t s, int n, const W *t, W *y) {
+    if (n <= 0) return;
+    k_add<W><<<blocks_for(n, kB), kB, 0, s>>>(n, t, y);
+}
+template void ildl_zero<double>(hipStream_t, int, double *);
+template void ildl_zero<float>(hipStream_t, int, float *);
+template void ildl_add<double>(hipStream_t, int, const double *, double *);
+template void ildl_add<float>(hipStream_t, int, const float *, float *);
 template <typename T, typename W>
 void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y) {
     if (m <= 0) return;

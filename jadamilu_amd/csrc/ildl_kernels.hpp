@@ -1,6 +1,11 @@
 // ildl_kernels.hpp -- gfx950 kernels of the multilevel ILDL apply.
 //
-// Per level (src/ilupack/symAMGsol.c:173-628 restated):
+// Per level (src/ilupack/symAMGsol.c:173-628 restated), with the coupling
+// block F^T = A_CB (the reference's structure):
+//   gather+scale -> y_B = B^-1 b_B (forward L_B sweep, D^-1, backward L_B^T
+//   sweep) -> coarse rhs b_C - F^T y_B -> [next level] -> x_B = y_B -
+//   B^-1 F x_C (a second solve with B) -> scatter+scale;
+// with the factor's own coupling rows L_CB instead:
 //   gather+scale -> forward L_B sweep -> coarse rhs (C rows) -> D^-1 ->
 //   [next level] -> B rows minus L_CB^T z_C -> backward L_B^T sweep ->
 //   scatter+scale.
@@ -69,6 +74,12 @@ void ildl_sub_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr, c
                      const T *val, const W *x, W *y);
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x);
+// x = 0 (a kernel, not a memset: the apply may be captured in a HIP graph)
+template <typename W>
+void ildl_zero(hipStream_t s, int n, W *x);
+// y += t
+template <typename W>
+void ildl_add(hipStream_t s, int n, const W *t, W *y);
 // y = Minv x, Minv m x m row-major
 template <typename T, typename W>
 void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y);

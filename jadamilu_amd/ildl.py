@@ -22,7 +22,7 @@ def _p(a, t):
 
 class IldlFactor:
     def __init__(self, n, ia, ja, a, *, shift=0.0, droptol=1e-3, condest=5.0, spd=False,
-                 single=False, ordering=2, dense_max=1500):
+                 single=False, ordering=2, dense_max=1500, coupling=0):
         L = lib()
         self._L = L
         ia = np.ascontiguousarray(ia, dtype=np.int32)
@@ -31,7 +31,7 @@ class IldlFactor:
         err = C.c_int(0)
         self.h = L.jdhip_ildl_factor(n, _p(ia, C.c_int), _p(ja, C.c_int), _p(a, C.c_double),
                                      shift, droptol, condest, int(spd), int(single), ordering,
-                                     dense_max, C.byref(err))
+                                     dense_max, int(coupling), C.byref(err))
         self.ierr = err.value
         self.n = n
         self.single = single
@@ -86,4 +86,5 @@ class IldlFactor:
         minv = np.zeros(m.value * m.value)
         if m.value:
             L.jdhip_ildl_dense(self.h, _p(minv, C.c_double))
-        return {"levels": levels, "m": m.value, "minv": minv.reshape(m.value, m.value)}
+        return {"levels": levels, "m": m.value, "minv": minv.reshape(m.value, m.value),
+                "coupling": L.jdhip_ildl_coupling(self.h)}

@@ -3,12 +3,16 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Sequential restatement of the multilevel
  * forward/backward substitution of DSYMAMGsol (src/ilupack/symAMGsol.c:
- * 110-631) for 1x1 pivots: per level scale+permute, forward sweep with the
- * unit lower factor L_B (sympiluclsol.F), coarse right-hand side with L_CB
- * (CSRmatvec.c), D^-1, recursion on the Schur complement, L_CB^T product
- * (CSRmattvec.c), backward sweep with L_B^T (sympilucusol.F), unpermute +
- * scale; the final level is an explicit dense inverse.  It consumes the
- * factor exported through the product's jdhip_ildl_level_data().
+ * 110-631) for 1x1 pivots.  Per level: scale+permute; with the coupling
+ * block F^T = A_CB (the reference's structure) y_B = B^-1 b_B (SYMPILUCSOL,
+ * sympilucsol.F:104-201: forward sweep with L_B, D^-1, backward sweep with
+ * L_B^T), coarse right-hand side b_C - F^T y_B (CSRmattvec.c), recursion on
+ * the Schur complement, then x_B = y_B - B^-1 F x_C (CSRmatvec.c and a
+ * second SYMPILUCSOL, symAMGsol.c:540-600); with L_CB instead the block
+ * LDL^T substitution (forward sweep, coarse rhs with L_CB, D^-1, recursion,
+ * L_CB^T product, backward sweep); unpermute + scale.  The final level is an
+ * explicit dense inverse.  It consumes the factor exported through the
+ * product's jdhip_ildl_level_data().
  */
 #include <stdlib.h>
 #include <string.h>
@@ -58,9 +62,73 @@ static void jacobi_lower(int nB, const int *ptr, const int *idx, const double *v
     free(xn);
 }
 
+/* x_B = L_B^-T D^-1 L_B^-1 x_B in place (exact, or in sweep mode) */
+static void solve_b(const jdo_ildl_level *V, double *x)
+{
+    const int nB = V->nB;
+    if (g_sweeps > 0) jacobi_lower(nB, V->lb_ptr, V->lb_idx, V->lb_val, x, 0);
+    else for (int i = 0; i < nB; i++) {
+        double s = 0.0;
+        for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) s += V->lb_val[k] * x[V->lb_idx[k]];
+        x[i] -= s;
+    }
+    for (int b = 0; b < nB; b++) x[b] /= V->d[b];
+    if (g_sweeps > 0) jacobi_lower(nB, V->lb_ptr, V->lb_idx, V->lb_val, x, 1);
+    else for (int i = nB - 1; i >= 0; i--) {
+        double xi = x[i];
+        for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) x[V->lb_idx[k]] -= V->lb_val[k] * xi;
+    }
+}
+
+/* coupling block of the factor being applied: 0 F^T = A_CB, 1 L_CB */
+static int g_coupling = 1;
+
+static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const double *minv,
+                        const double *in, double *out);
+
+static void level_apply_f(int l, int nlev, const jdo_ildl_level *L, int m, const double *minv,
+                          const double *in, double *out)
+{
+    const jdo_ildl_level *V = &L[l];
+    const int n = V->n, nB = V->nB, nC = n - nB;
+    double *buf = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double *zc = (double *)malloc(sizeof(double) * (nC > 0 ? nC : 1));
+    double *t = (double *)malloc(sizeof(double) * (nB > 0 ? nB : 1));
+    for (int i = 0; i < n; i++) {
+        int p = V->perm[i];
+        buf[i] = V->scal[p] * in[p];
+    }
+    solve_b(V, buf); /* y_B */
+    for (int c = 0; c < nC; c++) {
+        double s = 0.0;
+        for (int k = V->lc_ptr[c]; k < V->lc_ptr[c + 1]; k++) s += V->lc_val[k] * buf[V->lc_idx[k]];
+        buf[nB + c] -= s;
+    }
+    if (nC > 0) {
+        level_apply(l + 1, nlev, L, m, minv, buf + nB, zc);
+        for (int b = 0; b < nB; b++) t[b] = 0.0;
+        for (int c = 0; c < nC; c++)
+            for (int k = V->lc_ptr[c]; k < V->lc_ptr[c + 1]; k++)
+                t[V->lc_idx[k]] += V->lc_val[k] * zc[c];
+        solve_b(V, t);
+        for (int b = 0; b < nB; b++) buf[b] -= t[b];
+    }
+    for (int i = 0; i < n; i++) {
+        int p = V->perm[i];
+        out[p] = V->scal[p] * (i < nB ? buf[i] : zc[i - nB]);
+    }
+    free(buf);
+    free(zc);
+    free(t);
+}
+
 static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const double *minv,
                         const double *in, double *out)
 {
+    if (l < nlev && g_coupling == 0) {
+        level_apply_f(l, nlev, L, m, minv, in, out);
+        return;
+    }
     if (l == nlev) {
         for (int i = 0; i < m; i++) {
             double s = 0.0;
@@ -109,6 +177,8 @@ static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const d
     free(buf);
     free(zc);
 }
+
+void jdo_ildl_set_coupling(int c) { g_coupling = c ? 1 : 0; }
 
 void jdo_ildl_apply(int nlev, const jdo_ildl_level *levels, int m, const double *minv,
                     const double *r, double *z)

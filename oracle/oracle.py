@@ -75,6 +75,8 @@ def lib():
         L.jdo_ildl_apply.argtypes = [C.c_int, C.c_void_p, C.c_int, _d, _d, _d]
         L.jdo_ildl_set_sweeps.argtypes = [C.c_int]
         L.jdo_ildl_set_sweeps.restype = None
+        L.jdo_ildl_set_coupling.argtypes = [C.c_int]
+        L.jdo_ildl_set_coupling.restype = None
         L.jdo_set_psolve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
@@ -238,6 +240,7 @@ def pjd_ildl(n, ia, ja, a, factor, *, neig, isearch=0, sigma=0.0, ninit=0, madsp
     replaces the triangular solves by that many Jacobi sweeps."""
     L = lib()
     L.jdo_ildl_set_sweeps(int(sweeps))
+    L.jdo_ildl_set_coupling(int(factor.get("coupling", 1)))
     arr, minv, keep = _levels_struct(factor)
     ctx = _IldlCtx(len(factor["levels"]), C.cast(arr, C.c_void_p), factor["m"],
                    minv.ctypes.data_as(_d))
@@ -297,6 +300,7 @@ def ildl_apply(factor, r, sweeps=0):
     r = np.ascontiguousarray(r, dtype=np.float64)
     z = np.zeros_like(r)
     L.jdo_ildl_set_sweeps(int(sweeps))
+    L.jdo_ildl_set_coupling(int(factor.get("coupling", 1)))
     L.jdo_ildl_apply(len(levs), arr, factor["m"], minv.ctypes.data_as(_d), _p(r, C.c_double),
                      _p(z, C.c_double))
     L.jdo_ildl_set_sweeps(0)

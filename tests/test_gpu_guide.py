@@ -22,7 +22,7 @@ torch = pytest.importorskip("torch")
 N = 1000
 # set-ups whose fill the host factor does not reproduce (same list as the
 # CPU twin tests/test_guide_setups.py)
-FILL_OPEN = {"EXAMPLE1", "LISTING5_ADAPTIVE", "LISTING5_STATIC"}
+FILL_OPEN = {"EXAMPLE1"}
 
 
 def _dev():

@@ -35,20 +35,23 @@ def _apply_dev(ctx, r):
     return zd.cpu().numpy()
 
 
+@pytest.mark.parametrize("coupling", [0, 1])
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("gen,shift,spd", [
     (lambda: M.laplace((18, 17, 16)), 0.0, True),
     (lambda: M.anderson(14, 16.5, 4), 0.0, False),
     (lambda: M.tridiag(1000, np.arange(1000) * 1.0, 5.0), 0.0, False),
 ])
-def test_device_apply_vs_oracle(gen, shift, spd, single):
+def test_device_apply_vs_oracle(gen, shift, spd, single, coupling):
+    """coupling 0: F^T blocks and two solves with B per level (the
+    reference's DSYMAMGsol); 1: L_CB blocks (block LDL^T substitution)."""
     from jadamilu_amd.ildl import IldlFactor
     from jadamilu_amd.jd import Context
     _dev()
     ia, ja, a = gen()
     n = ia.size - 1
     f = IldlFactor(n, ia, ja, a, shift=shift, droptol=1e-3, spd=spd, single=single,
-                   dense_max=300)
+                   dense_max=300, coupling=coupling)
     assert f.ierr == 0
     ex = f.export()
     ctx = Context(0)

@@ -44,9 +44,6 @@ _ISEARCH = {"EXAMPLE1": 2, "EXAMPLE2": 0, "EXAMPLE11": 0, "LISTING5_ADAPTIVE": 0
 _OPEN = {
     "EXAMPLE1": "indefinite tridiagonal: the reference pairs rows into 2x2 pivots from a "
                 "maximum-product matching (sympermMC64amd.c, symwm.c); fill differs",
-    "LISTING5_ADAPTIVE": "near-singular SPD tridiagonal: the deferral pattern of the "
-                         "inverse-norm estimate differs from sympiluc.F's",
-    "LISTING5_STATIC": "as LISTING5_ADAPTIVE",
 }
 
 
