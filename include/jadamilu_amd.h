@@ -227,9 +227,39 @@ int jdhip_ildl_level_sizes(jdhip_ildl *f, int lev, int *n, int *nB, long *nnz_lb
 int jdhip_ildl_level_data(jdhip_ildl *f, int lev, int *perm, double *scal, int *lb_ptr,
                           int *lb_idx, double *lb_val, int *lc_ptr, int *lc_idx, double *lc_val,
                           double *d);
+/* block diagonal D of level lev: d[nB] its diagonal, and for 2x2 pivots
+ * (the reference's 2x2 blocks, sympiluc.F:1042-1200) dpair[b] = the other
+ * row of b's block (b-1 or b+1) with doff[b] = the block's off-diagonal
+ * entry; dpair[b] = -1, doff[b] = 0 for a 1x1 pivot.  0, or -1 for a bad
+ * level. */
+int jdhip_ildl_level_pivots(jdhip_ildl *f, int lev, double *d, double *doff, int *dpair);
 /* dense final level: explicit inverse (m x m, row-major) */
 int jdhip_ildl_dense(jdhip_ildl *f, double *minv);
 int jdhip_ildl_stats(jdhip_ildl *f, double *fill, int *nneg, double *condest, double *seconds);
+/* Persisted factor: everything the apply reads (the reference keeps it in
+ * memory as the DAMGlevelmat list, src/include/ilupack.h:197-211, for the
+ * lifetime of one process; here it can also be written to a file and read
+ * back by a later run, then attached with jdhip_ildl_attach and used with
+ * ICNTL(2) = -1).  Binary, little-endian, versioned, with a checksum; the
+ * layout is described in jadamilu_amd/csrc/persist.cpp.  save: 0, -1 bad
+ * arguments, -2 I/O error.  load: the handle, or NULL with *ierr = -2 (I/O)
+ * or -3 (not a factor file, version or checksum mismatch, inconsistent
+ * structure). */
+int jdhip_ildl_save(jdhip_ildl *f, const char *path);
+jdhip_ildl *jdhip_ildl_load(const char *path, int *ierr);
+
+/* MatrixMarket coordinate files (real; symmetric, or general holding a
+ * symmetric matrix, of which the upper triangle is read; pattern files get
+ * values 1) to the reference's input: 1-based upper-triangle CSR with every
+ * diagonal entry stored (an absent diagonal entry becomes an explicit 0) and
+ * duplicates summed.  info: order and stored entries; read: fills caller
+ * arrays ia[n+1], ja[nnz], a[nnz] (nnz_cap >= the info count); write: a
+ * symmetric file (lower triangle, 17 significant digits) from such a CSR.
+ * 0 on success, -1 bad arguments, -2 I/O error, -3 malformed file, -4
+ * unsupported kind (complex, Hermitian, skew), -5 size mismatch. */
+int jdhip_mm_info(const char *path, int *n, long *nnz_upper);
+int jdhip_mm_read(const char *path, int n, long nnz_cap, int *ia, int *ja, double *a);
+int jdhip_mm_write(const char *path, int n, const int *ia, const int *ja, const double *a);
 /* upload f as the preconditioner of ctx (sweeps as "ildl_sweeps") */
 int jdhip_ildl_attach(jdhip_ctx *ctx, jdhip_ildl *f, int sweeps);
 /* dependency levels of the uploaded factor and algorithmic bytes per apply */

@@ -86,11 +86,17 @@ SIGNATURES = {
     "jdhip_ildl_factor": (_v, [C.c_int, _i, _i, _d, C.c_double, C.c_double, C.c_double, C.c_int,
                                C.c_int, C.c_int, C.c_int, C.c_int, _i]),
     "jdhip_ildl_coupling": (C.c_int, [_v]),
+    "jdhip_ildl_save": (C.c_int, [_v, C.c_char_p]),
+    "jdhip_ildl_load": (_v, [C.c_char_p, _i]),
+    "jdhip_mm_info": (C.c_int, [C.c_char_p, _i, _l]),
+    "jdhip_mm_read": (C.c_int, [C.c_char_p, C.c_int, C.c_long, _i, _i, _d]),
+    "jdhip_mm_write": (C.c_int, [C.c_char_p, C.c_int, _i, _i, _d]),
     "jdhip_ildl_free": (None, [_v]),
     "jdhip_ildl_nlevels": (C.c_int, [_v, _i]),
     "jdhip_ildl_level_sizes": (C.c_int, [_v, C.c_int, _i, _i, _l, _l]),
     "jdhip_ildl_level_data": (C.c_int, [_v, C.c_int, _i, _d, _i, _i, _d, _i, _i, _d, _d]),
     "jdhip_ildl_dense": (C.c_int, [_v, _d]),
+    "jdhip_ildl_level_pivots": (C.c_int, [_v, C.c_int, _d, _d, _i]),
     "jdhip_ildl_stats": (C.c_int, [_v, _d, _i, _d, _d]),
     "jdhip_ildl_attach": (C.c_int, [_v, _v, C.c_int]),
     "jdhip_ildl_device_info": (C.c_int, [_v, _i, _i, _l]),

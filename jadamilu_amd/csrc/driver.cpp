@@ -1131,9 +1131,7 @@ int jdhip_ildl_time_sweep(jdhip_ctx *ctx, int reps, double *avg_ms, long *alg_by
 
 }  // extern "C"
 
-struct jdhip_ildl {
-    jdhip::IldlHost h;
-};
+#include "ildl_handle.hpp"
 
 namespace {
 template <typename T>
@@ -1212,6 +1210,20 @@ int jdhip_ildl_level_data(jdhip_ildl *f, int lev, int *perm, double *scal, int *
     else
         export_level(f->h.fd, lev, perm, scal, lb_ptr, lb_idx, lb_val, lc_ptr, lc_idx, lc_val, d);
     return 0;
+}
+
+int jdhip_ildl_level_pivots(jdhip_ildl *f, int lev, double *d, double *doff, int *dpair) {
+    auto get = [&](const auto &ml) {
+        if (lev < 0 || lev >= (int)ml.levels.size()) return -1;
+        const auto &L = ml.levels[lev];
+        for (int b = 0; b < L.nB; ++b) {
+            d[b] = (double)L.d[b];
+            doff[b] = L.doff.empty() ? 0.0 : (double)L.doff[b];
+            dpair[b] = L.dpair.empty() ? -1 : L.dpair[b];
+        }
+        return 0;
+    };
+    return f->h.single ? get(f->h.ff) : get(f->h.fd);
 }
 
 int jdhip_ildl_dense(jdhip_ildl *f, double *minv) {

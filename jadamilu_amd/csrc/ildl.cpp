@@ -214,6 +214,10 @@ struct DevLevel {
     int n = 0, nB = 0, nC = 0;
     int *perm = nullptr;
     double *scal = nullptr, *dinv = nullptr;  // per row, double on both paths
+    // 2x2 pivots (null if the level has none): off-diagonal entry of each
+    // row's block of D^-1 and the partner row (storage numbering, -1: 1x1)
+    double *doinv = nullptr;
+    int *dpair = nullptr;
     int *lb_ptr = nullptr, *lb_idx = nullptr;
     T *lb_val = nullptr;
     int *lt_ptr = nullptr, *lt_idx = nullptr;
@@ -291,9 +295,31 @@ class IldlImpl : public IldlImplBase {
             for (int t = 0; t < n; ++t) permN[t] = perm[q[t]];
             D.perm = track(dev_upload(permN));
             D.scal = track(dev_upload(scal));
-            std::vector<double> dinv(nB);
-            for (int t = 0; t < nB; ++t) dinv[t] = 1.0 / (double)L.d[q[t]];
+            // D^-1 per stored row: 1 / d for a 1x1 pivot; for a 2x2 block
+            // [a o; o c] the row's entries of [c -o; -o a] / (ac - o^2) and the
+            // partner's storage index
+            std::vector<double> dinv(nB), doinv(nB, 0.0);
+            std::vector<int> dpair_s(nB, -1);
+            bool any_pair = false;
+            for (int t = 0; t < nB; ++t) {
+                const int b = q[t];
+                const int pb = L.dpair.empty() ? -1 : L.dpair[b];
+                if (pb < 0) {
+                    dinv[t] = 1.0 / (double)L.d[b];
+                } else {
+                    any_pair = true;
+                    const double a = (double)L.d[b], o = (double)L.doff[b], c = (double)L.d[pb];
+                    const double det = a * c - o * o;
+                    dinv[t] = c / det;
+                    doinv[t] = -o / det;
+                    dpair_s[t] = qinv[pb];
+                }
+            }
             D.dinv = track(dev_upload(dinv));
+            if (any_pair) {
+                D.doinv = track(dev_upload(doinv));
+                D.dpair = track(dev_upload(dpair_s));
+            }
             // L_B and L_CB in the storage numbering
             std::vector<int> bp(nB + 1, 0), bi, cp(nC + 1, 0), ci;
             std::vector<T> bv, cv;
@@ -721,12 +747,12 @@ class IldlImpl : public IldlImplBase {
             tri_solve(D, true);
             if (fform()) {
                 // y_B = B^-1 b_B, then b_C -= F^T y_B
-                ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
+                dscale(D, D.buf);
                 tri_solve(D, false);
                 if (D.nC > 0) sub_launch(D, false, D.wc, D.buf, D.buf + D.nB);
             } else {
                 if (D.nC > 0) sub_launch(D, false, D.wc, D.buf, D.buf + D.nB);
-                ildl_dscale<W>(s, D.nB, D.dinv, D.buf);
+                dscale(D, D.buf);
             }
         }
         const W *zc = nullptr;
@@ -744,7 +770,7 @@ class IldlImpl : public IldlImplBase {
                     ildl_zero<W>(s, D.nB, t);
                     sub_launch(D, true, D.wct, zc, t);
                     tri_solve(D, true, t);
-                    ildl_dscale<W>(s, D.nB, D.dinv, t);
+                    dscale(D, t);
                     tri_solve(D, false, t);
                     ildl_add<W>(s, D.nB, t, D.buf);
                 }
@@ -759,6 +785,11 @@ class IldlImpl : public IldlImplBase {
     }
 
     bool fform() const { return f_.coupling == 0; }
+    // x_B = D^-1 x_B (1x1 and 2x2 pivots)
+    void dscale(const DevLevel<T, W> &D, W *x) {
+        if (D.dpair) ildl_dscale2<W>(ctx_->stream, D.nB, D.dinv, D.doinv, D.dpair, x);
+        else ildl_dscale<W>(ctx_->stream, D.nB, D.dinv, x);
+    }
     int nlevels() const override { return (int)f_.levels.size(); }
     int dense_m() const override { return f_.m; }
     void level_sizes(int lev, int *n, int *nB, long *nnz_lb, long *nnz_lc) const override {
@@ -901,6 +932,8 @@ static IldlMultilevel<float> round_to_float(const IldlMultilevel<double> &d) {
         F.lc_idx = L.lc_idx;
         F.lc_val.assign(L.lc_val.begin(), L.lc_val.end());
         F.d.assign(L.d.begin(), L.d.end());
+        F.doff.assign(L.doff.begin(), L.doff.end());
+        F.dpair = L.dpair;
         f.levels.push_back(std::move(F));
     }
     f.m = d.m;

@@ -1,6 +1,7 @@
 // ildl_factor.cpp -- host multilevel inverse-based incomplete LDL^T (see
 // ildl_factor.hpp for the reference routines each step restates).
 #include "ildl_factor.hpp"
+#include "matching.hpp"
 #include "ordering.hpp"
 
 #include <algorithm>
@@ -120,21 +121,40 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     const int n = A.n;
     IldlLevel<T> &L = out.L;
     L.n = n;
-    // ---- scaling (powers of two) ----
-    std::vector<double> s(n);
-    for (int i = 0; i < n; ++i) {
-        double m = 0.0;
-        for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) m = std::max(m, std::fabs(A.val[k]));
-        s[i] = (m > 0.0) ? pow2_near(1.0 / std::sqrt(m)) : 1.0;
-    }
-    // ---- ordering ----
     const auto t_start = std::chrono::steady_clock::now();
+    // ---- scaling (powers of two) and ordering ----
+    std::vector<double> s(n);
     std::vector<int> order;
-    if (prm.ordering == 2) order = amd_order(n, A.ptr.data(), A.idx.data());
-    else if (prm.ordering == 1) order = rcm_order(A);
-    else {
-        order.resize(n);
-        std::iota(order.begin(), order.end(), 0);
+    std::vector<char> pair_start(n, 0);  // position k starts a matched 2x2 block
+    if (prm.matching && prm.ordering == 2) {
+        // the reference's preprocessing of every level: SYMSCALE, symmetric
+        // maximum weighted matching with its scaling, 1x1/2x2 blocks, AMD on
+        // the compressed graph (matching.cpp)
+        const std::vector<double> s1 = symscale(A);
+        SymCsr A1 = A;
+        for (int i = 0; i < n; ++i)
+            for (int t = A.ptr[i]; t < A.ptr[i + 1]; ++t)
+                A1.val[t] = s1[i] * A.val[t] * s1[A.idx[t]];
+        const MatchResult mr = mwm_symmetric(A1);
+        for (int i = 0; i < n; ++i)
+            for (int t = A1.ptr[i]; t < A1.ptr[i + 1]; ++t)
+                A1.val[t] *= mr.scale[i] * mr.scale[A1.idx[t]];
+        std::vector<int> wm;
+        const int nsingle = symwm_pairs(A1, mr.match, wm);
+        order = pair_amd_order(A1, wm, nsingle, pair_start);
+        for (int i = 0; i < n; ++i) s[i] = s1[i] * mr.scale[i];
+    } else {
+        for (int i = 0; i < n; ++i) {
+            double m = 0.0;
+            for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) m = std::max(m, std::fabs(A.val[k]));
+            s[i] = (m > 0.0) ? pow2_near(1.0 / std::sqrt(m)) : 1.0;
+        }
+        if (prm.ordering == 2) order = amd_order(n, A.ptr.data(), A.idx.data());
+        else if (prm.ordering == 1) order = rcm_order(A);
+        else {
+            order.resize(n);
+            std::iota(order.begin(), order.end(), 0);
+        }
     }
     std::vector<int> pos(n);
     for (int k = 0; k < n; ++k) pos[order[k]] = k;
@@ -193,7 +213,13 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     std::vector<std::pair<int, T>> kept;
     int nB = 0, nC = 0;
     double maxcond = 1.0;
-    auto acc = [&](int i, T v, int k) {
+    // second accumulator: column k+1 of a 2x2 pivot candidate
+    std::vector<T> wv2(n, T(0));
+    std::vector<int> wmark2(n, -1), wlist2;
+    // 2x2 pivots: off-diagonal entry of the block and the partner row
+    std::vector<T> doff(n, T(0));
+    std::vector<int> partner(n, -1);
+    auto acc1 = [&](int i, T v, int k) {
         if (wmark[i] != k) {
             wmark[i] = k;
             wv[i] = v;
@@ -202,36 +228,287 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             wv[i] += v;
         }
     };
-    for (int k = 0; k < n; ++k) {
-        wlist.clear();
-        T wk = T(0);
-        for (int t = ap[k]; t < ap[k + 1]; ++t) {
-            int i = ai[t];
-            if (i == k) wk = av[t];
-            else if (status[i] != 1) acc(i, av[t], k);
+    auto acc2 = [&](int i, T v, int k) {
+        if (wmark2[i] != k) {
+            wmark2[i] = k;
+            wv2[i] = v;
+            wlist2.push_back(i);
+        } else {
+            wv2[i] += v;
         }
-        for (long e = rhead[k]; e >= 0; e = rnext[e]) {
-            int p = rcol[e];
-            T lkp = rval[e];
-            T coef = lkp * d[p];
-            wk -= coef * lkp;
-            // rows passed so far: deferred ones stay in the column's deferred
-            // list (row k itself is kept ahead of the pointer until its own
-            // accept/defer decision is made)
-            while (cur[p] < cend[p] && crow[cur[p]] < k) {
-                int r = crow[cur[p]];
-                if (status[r] == 2) {
-                    dnext.push_back(dhead[p]);
-                    drow.push_back(r);
-                    dval.push_back(cval[cur[p]]);
-                    dhead[p] = (long)drow.size() - 1;
+    };
+    // Column j of the not yet factored part: A(:,j) minus the updates of the
+    // accepted pivots (1x1, or 2x2 blocks D_pq), into accumulator 1 or 2.
+    // advance: move the column pointers past the rows before j (the deferred
+    // ones go to the column's deferred list); the 2x2 partner column is
+    // gathered without advancing.  Returns the updated diagonal entry.
+    auto gather = [&](int j, bool second) -> T {
+        auto acc = [&](int i, T v) {
+            if (second) acc2(i, v, j);
+            else acc1(i, v, j);
+        };
+        T wj = T(0);
+        for (int t = ap[j]; t < ap[j + 1]; ++t) {
+            const int i = ai[t];
+            if (i == j) wj = av[t];
+            else if (status[i] != 1) acc(i, av[t]);
+        }
+        auto apply_col = [&](int p, T coef, T ljp) {
+            wj -= coef * ljp;
+            if (!second) {
+                while (cur[p] < cend[p] && crow[cur[p]] < j) {
+                    const int r = crow[cur[p]];
+                    if (status[r] == 2) {
+                        dnext.push_back(dhead[p]);
+                        drow.push_back(r);
+                        dval.push_back(cval[cur[p]]);
+                        dhead[p] = (long)drow.size() - 1;
+                    }
+                    cur[p]++;
                 }
-                cur[p]++;
             }
-            for (long t = cur[p]; t < cend[p]; ++t)
-                if (crow[t] != k) acc(crow[t], -coef * cval[t], k);
+            for (long t = cur[p]; t < cend[p]; ++t) {
+                const int r = crow[t];
+                if (r != j && status[r] != 1) acc(r, -coef * cval[t]);
+            }
             for (long t = dhead[p]; t >= 0; t = dnext[t])
-                if (drow[t] != k) acc(drow[t], -coef * dval[t], k);
+                if (drow[t] != j) acc(drow[t], -coef * dval[t]);
+        };
+        for (long e = rhead[j]; e >= 0; e = rnext[e]) {
+            const int p = rcol[e];
+            const T l0 = rval[e];
+            if (partner[p] < 0) {
+                apply_col(p, l0 * d[p], l0);
+            } else if (partner[p] == p + 1) {
+                // 2x2 block (p, p+1): coefficients D [L(j,p) L(j,p+1)]^T
+                T l1 = T(0);
+                const long nx = rnext[e];
+                if (nx >= 0 && rcol[nx] == p + 1) {
+                    l1 = rval[nx];
+                    e = nx;
+                }
+                apply_col(p, d[p] * l0 + doff[p] * l1, l0);
+                apply_col(p + 1, doff[p] * l0 + d[p + 1] * l1, l1);
+            } else {
+                // second column of a block whose first entry is absent in row j
+                const int q = p - 1;
+                apply_col(q, doff[q] * l0, T(0));
+                apply_col(p, d[p] * l0, l0);
+            }
+        }
+        return wj;
+    };
+    auto store_col = [&](int k, const std::vector<std::pair<int, T>> &col) {
+        cbeg[k] = (long)crow.size();
+        for (auto &kv : col) {
+            crow.push_back(kv.first);
+            cval.push_back(kv.second);
+            // append to row list of kv.first (keeps column order increasing)
+            rnext.push_back(-1);
+            rcol.push_back(k);
+            rval.push_back(kv.second);
+            const long id = (long)rcol.size() - 1;
+            if (rtail[kv.first] < 0) rhead[kv.first] = id;
+            else rnext[rtail[kv.first]] = id;
+            rtail[kv.first] = id;
+        }
+        cend[k] = (long)crow.size();
+        cur[k] = cbeg[k];
+    };
+    std::vector<std::pair<int, T>> kept2;
+    std::vector<int> urows;
+    for (int k = 0; k < n;) {
+        wlist.clear();
+        const T wk = gather(k, false);
+        // ---- 1x1 or 2x2 pivot (sympiluc.F:1042-1200) ----
+        // 2x2 blocks are tried where the matching paired rows k, k+1; the
+        // 2x2 pivot is taken when it is more diagonally dominant than the
+        // 1x1 pivot would be
+        int mode = 0;
+        T wk1 = T(0), b = T(0);
+        double a2i11 = 0, a2i12 = 0, a2i22 = 0, mum = 0;
+        double lmax1 = 0.0;
+        for (int i : wlist) lmax1 = std::max(lmax1, std::fabs((double)wv[i]));
+        if (k + 1 < n && pair_start[k] && status[k + 1] == 0) {
+            b = (wmark[k + 1] == k) ? wv[k + 1] : T(0);
+            if (b != T(0)) {
+                wlist2.clear();
+                wk1 = gather(k + 1, true);
+                const double A11 = (double)wk, A12 = (double)b, A22 = (double)wk1;
+                const double det = A11 * A22 - A12 * A12;
+                const double mup = (A11 != 0.0) ? lmax1 / std::fabs(A11) : 0.0;
+                if (det != 0.0 && std::fabs(A12) > 0.1 * std::min(std::fabs(A11), std::fabs(A22))) {
+                    double mx = 0.0;
+                    for (int i : wlist) {
+                        if (i == k + 1) continue;
+                        const double w1 = (double)wv[i];
+                        const double w2 = (wmark2[i] == k + 1) ? (double)wv2[i] : 0.0;
+                        mx = std::max(mx, std::fabs(A22 * w1 - A12 * w2) + std::fabs(-A12 * w1 + A11 * w2));
+                    }
+                    for (int i : wlist2) {
+                        if (i == k || wmark[i] == k) continue;
+                        const double w2 = (double)wv2[i];
+                        mx = std::max(mx, std::fabs(A12 * w2) + std::fabs(A11 * w2));
+                    }
+                    mum = mx / std::fabs(det);
+                    const double mup_eff = (A11 == 0.0) ? 10.0 * (mum + 1.0) : mup;
+                    if (mup_eff > mum) {
+                        mode = 1;
+                        a2i11 = A22 / det;
+                        a2i12 = -A12 / det;
+                        a2i22 = A11 / det;
+                    }
+                }
+            }
+        }
+        if (mode == 1) {
+            // ---- 2x2 pivot (k, k+1) ----
+            // rows of U(k:k+1, :) outside the block, with both entries
+            urows.clear();
+            for (int i : wlist)
+                if (i != k + 1) urows.push_back(i);
+            for (int i : wlist2)
+                if (i != k && wmark[i] != k) urows.push_back(i);
+            auto u1 = [&](int i) { return (wmark[i] == k) ? (double)wv[i] : 0.0; };
+            auto u2 = [&](int i) { return (wmark2[i] == k + 1) ? (double)wv2[i] : 0.0; };
+            const double U11 = a2i11, U12 = a2i12, U21 = a2i12, U22 = a2i22;
+            bool skip = false;
+            // first estimate with the four sign pairs (sympiluc.F:1513-1600)
+            double x = 0, x2 = 0;
+            {
+                const double sk[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+                double X[4][2], mu[4] = {0, 0, 0, 0}, nu[4] = {0, 0, 0, 0};
+                for (int c = 0; c < 4; ++c) {
+                    const double r1 = sk[c][0] - est1[k], r2 = sk[c][1] - est1[k + 1];
+                    X[c][0] = r1 * U11 + r2 * U21;
+                    X[c][1] = r1 * U12 + r2 * U22;
+                }
+                for (int i : urows) {
+                    const double a1 = u1(i), a2 = u2(i);
+                    for (int c = 0; c < 4; ++c) {
+                        const double ax = std::fabs(est1[i] + a1 * X[c][0] + a2 * X[c][1]);
+                        nu[c] = std::max(nu[c], ax);
+                        mu[c] += ax;
+                    }
+                }
+                // ties as the reference: (+,+), (+,-), (-,+), then (-,-)
+                int cb = 3;
+                if (mu[0] >= mu[3] && mu[0] >= mu[1] && mu[0] >= mu[2]) cb = 0;
+                else if (mu[1] >= mu[3] && mu[1] >= mu[0] && mu[1] >= mu[2]) cb = 1;
+                else if (mu[2] >= mu[3] && mu[2] >= mu[0] && mu[2] >= mu[1]) cb = 2;
+                x = X[cb][0];
+                x2 = X[cb][1];
+                const double axp = std::max(std::fabs(sk[cb][0] - est1[k]),
+                                            std::fabs(sk[cb][1] - est1[k + 1]));
+                if (std::max(nu[cb], axp) > condest) skip = true;
+            }
+            // peaks and first-estimate update (sympiluc.F:1642-1697)
+            const double e1k1_save = est1[k + 1];
+            const int pk1_save = peaks[k + 1];
+            {
+                int m1 = peaks[k], m2 = peaks[k + 1];
+                if (std::fabs(est1[k]) <= (double)m1) ++m1;
+                if (std::fabs(est1[k + 1]) <= (double)m2 && !skip) ++m2;
+                peaks[k] = m1;
+                if (!skip) peaks[k + 1] = m2;
+                if (!skip) {
+                    for (int i : urows) {
+                        const double yp = u1(i) * x, ym = u2(i) * x2;
+                        if (std::fabs(yp) * peaks[i] >= std::fabs(est1[i]) * m1) peaks[i] = m1;
+                        if (std::fabs(ym) * peaks[i] >= std::fabs(est1[i]) * m2) peaks[i] = m2;
+                        est1[i] += yp + ym;
+                    }
+                    est1[k] = std::max(std::fabs(1.0 - est1[k]), std::fabs(-1.0 - est1[k]));
+                    est1[k + 1] =
+                        std::max(std::fabs(1.0 - est1[k + 1]), std::fabs(-1.0 - est1[k + 1]));
+                } else {
+                    est1[k] = 1.0;
+                }
+            }
+            // improved estimate (sympiluc.F:1883-2015)
+            if (!skip) {
+                const double sk[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+                double X[4][2], nu[4] = {0, 0, 0, 0};
+                int cnt[4] = {0, 0, 0, 0};
+                for (int c = 0; c < 4; ++c) {
+                    const double r1 = sk[c][0] - est2[k], r2 = sk[c][1] - est2[k + 1];
+                    X[c][0] = r1 * U11 + r2 * U21;
+                    X[c][1] = r1 * U12 + r2 * U22;
+                }
+                for (int i : urows) {
+                    const double a1 = u1(i), a2 = u2(i), ax = std::fabs(est2[i]);
+                    for (int c = 0; c < 4; ++c) {
+                        const double ay = std::fabs(est2[i] + a1 * X[c][0] + a2 * X[c][1]);
+                        nu[c] = std::max(nu[c], ay);
+                        if (ay > 2.0 * ax && ay > 0.5) ++cnt[c];
+                        if (2.0 * ay < ax && ax > 0.5) --cnt[c];
+                    }
+                }
+                // order of the reference: (+,+) j, (+,-) jj, (-,+) r, else (-,-) m
+                int cb = 3;
+                if (cnt[0] >= cnt[3] && cnt[0] >= cnt[1] && cnt[0] >= cnt[2]) cb = 0;
+                else if (cnt[1] >= cnt[3] && cnt[1] >= cnt[0] && cnt[1] >= cnt[2]) cb = 1;
+                else if (cnt[2] >= cnt[3] && cnt[2] >= cnt[0] && cnt[2] >= cnt[1]) cb = 2;
+                const double axp = std::max(std::fabs(sk[cb][0] - est2[k]),
+                                            std::fabs(sk[cb][1] - est2[k + 1]));
+                if (std::max(nu[cb], axp) > condest) skip = true;
+                if (!skip) {
+                    for (int i : urows) est2[i] += u1(i) * X[cb][0] + u2(i) * X[cb][1];
+                    est1[k] = std::max({est1[k], std::fabs(1.0 - est2[k]), std::fabs(-1.0 - est2[k])});
+                    est1[k + 1] = std::max(
+                        {est1[k + 1], std::fabs(1.0 - est2[k + 1]), std::fabs(-1.0 - est2[k + 1])});
+                } else {
+                    est1[k] = 1.0;
+                    for (int i : urows) est1[i] -= u1(i) * x + u2(i) * x2;  // undo
+                    est1[k + 1] = e1k1_save;
+                    peaks[k + 1] = pk1_save;
+                }
+            }
+            if (peaks[k] > 2) est1[k] = 2.0 * est1[k] / peaks[k];
+            if (est1[k] < 1.0) est1[k] = 1.0;
+            if (skip) {
+                // only row k is deferred; k+1 is considered on its own next
+                status[k] = 2;
+                cpos[k] = nC++;
+                ++k;
+                continue;
+            }
+            if (peaks[k + 1] > 2) est1[k + 1] = 2.0 * est1[k + 1] / peaks[k + 1];
+            if (est1[k + 1] < 1.0) est1[k + 1] = 1.0;
+            status[k] = status[k + 1] = 1;
+            bpos[k] = nB++;
+            bpos[k + 1] = nB++;
+            d[k] = wk;
+            d[k + 1] = wk1;
+            doff[k] = doff[k + 1] = b;
+            partner[k] = k + 1;
+            partner[k + 1] = k;
+            // pair dropping (sympiluc.F:2472-2538): (U(k,l), U(k+1,l)) with
+            // max |U(.,l)| Umax <= droptol / maxcondest, Umax the largest
+            // ||D^-1 U(k:k+1, m)||
+            maxcond = std::max({maxcond, est1[k], est1[k + 1]});
+            const double thr = prm.droptol / maxcond;
+            kept.clear();
+            kept2.clear();
+            for (int i : urows) {
+                const double a1 = u1(i), a2 = u2(i);
+                if (std::max(std::fabs(a1), std::fabs(a2)) * mum > thr) {
+                    kept.emplace_back(i, (T)(a1 * U11 + a2 * U21));
+                    kept2.emplace_back(i, (T)(a1 * U12 + a2 * U22));
+                }
+            }
+            auto by_row = [](const auto &x1, const auto &y1) { return x1.first < y1.first; };
+            std::sort(kept.begin(), kept.end(), by_row);
+            std::sort(kept2.begin(), kept2.end(), by_row);
+            store_col(k, kept);
+            store_col(k + 1, kept2);
+            nnz_stored += 2 * (long)kept.size() + 2;
+            if (nnz_limit > 0 && nnz_stored > 2 * nnz_limit) {
+                out.ierr = -3;
+                return;
+            }
+            k += 2;
+            continue;
         }
         bool skip = !(wk != T(0)) || !std::isfinite((double)wk);
         const double dk = (double)wk;
@@ -320,6 +597,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         if (skip) {
             status[k] = 2;
             cpos[k] = nC++;
+            ++k;
             continue;
         }
         status[k] = 1;
@@ -338,21 +616,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         }
         std::sort(kept.begin(), kept.end(),
                   [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
-        cbeg[k] = (long)crow.size();
-        for (auto &kv : kept) {
-            crow.push_back(kv.first);
-            cval.push_back(kv.second);
-            // append to row list of kv.first (keeps column order increasing)
-            rnext.push_back(-1);
-            rcol.push_back(k);
-            rval.push_back(kv.second);
-            long id = (long)rcol.size() - 1;
-            if (rtail[kv.first] < 0) rhead[kv.first] = id;
-            else rnext[rtail[kv.first]] = id;
-            rtail[kv.first] = id;
-        }
-        cend[k] = (long)crow.size();
-        cur[k] = cbeg[k];
+        store_col(k, kept);
         // entries produced so far (L_B and L_CB rows): a runaway guard at
         // twice the elbow space; the elbow limit itself applies to the
         // stored factor once the level is complete
@@ -361,6 +625,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             out.ierr = -3;
             return;
         }
+        ++k;
     }
     out.maxcond = maxcond;
     out.nC = nC;
@@ -375,10 +640,16 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     L.scal.assign(n, T(1));
     for (int i = 0; i < n; ++i) L.scal[i] = (T)s[i];
     L.d.assign(nB, T(0));
+    L.doff.assign(nB, T(0));
+    L.dpair.assign(nB, -1);
     std::vector<int> lbcnt(nB + 1, 0), lccnt(nC + 1, 0);
     for (int k = 0; k < n; ++k) {
         if (status[k] != 1) continue;
         L.d[bpos[k]] = d[k];
+        if (partner[k] >= 0) {
+            L.doff[bpos[k]] = doff[k];
+            L.dpair[bpos[k]] = bpos[partner[k]];
+        }
         for (long t = cbeg[k]; t < cend[k]; ++t) {
             int r = crow[t];
             if (status[r] == 1) lbcnt[bpos[r] + 1]++;
@@ -473,11 +744,33 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
                 if (status[j] == 2) sacc(cpos[j], (double)av[t]);
             }
             sacc(c, 0.0);
-            for (int t = L.lc_ptr[c]; t < L.lc_ptr[c + 1]; ++t) {
-                int b = L.lc_idx[t];
-                double coef = (double)L.lc_val[t] * (double)L.d[b];
+            // S(c, :) -= sum_b g_b L_CB(:, b)^T with g = D L_CB(c, :)^T (D with
+            // 1x1 and 2x2 blocks; the two columns of a block are adjacent)
+            auto col_update = [&](int b, double g) {
+                if (g == 0.0) return;
                 for (int u = lcc_ptr[b]; u < lcc_ptr[b + 1]; ++u)
-                    sacc(lcc_idx[u], -coef * (double)lcc_val[u]);
+                    sacc(lcc_idx[u], -g * (double)lcc_val[u]);
+            };
+            for (int t = L.lc_ptr[c]; t < L.lc_ptr[c + 1]; ++t) {
+                const int b = L.lc_idx[t];
+                const double l = (double)L.lc_val[t];
+                const int q = L.dpair[b];
+                if (q < 0) {
+                    col_update(b, l * (double)L.d[b]);
+                } else if (q == b + 1) {
+                    double l1 = 0.0;
+                    if (t + 1 < L.lc_ptr[c + 1] && L.lc_idx[t + 1] == b + 1) {
+                        l1 = (double)L.lc_val[t + 1];
+                        ++t;
+                    }
+                    const double o = (double)L.doff[b];
+                    col_update(b, (double)L.d[b] * l + o * l1);
+                    col_update(b + 1, o * l + (double)L.d[b + 1] * l1);
+                } else {
+                    const double o = (double)L.doff[b];
+                    col_update(q, o * l);
+                    col_update(b, (double)L.d[b] * l);
+                }
             }
             srow.clear();
             for (int j : slist)
@@ -739,6 +1032,52 @@ int dense_inverse(const SymCsr &S, bool spd, std::vector<double> &inv, int &nneg
 
 }  // namespace
 
+// Eigenvalues of the symmetric 2x2 block [a b; b c] (larger first).
+static void eig2(double a, double b, double c, double &l1, double &l2) {
+    const double tr = 0.5 * (a + c), df = 0.5 * (a - c), rt = std::sqrt(df * df + b * b);
+    l1 = tr + rt;
+    l2 = tr - rt;
+}
+
+// Number of negative eigenvalues of the level's block diagonal D.
+template <typename T>
+static int negative_pivots(const IldlLevel<T> &L) {
+    int neg = 0;
+    for (int b = 0; b < L.nB; ++b) {
+        const int q = L.dpair.empty() ? -1 : L.dpair[b];
+        if (q < 0) {
+            neg += (L.d[b] < T(0));
+        } else if (q == b + 1) {
+            double l1, l2;
+            eig2((double)L.d[b], (double)L.doff[b], (double)L.d[b + 1], l1, l2);
+            neg += (l1 < 0.0) + (l2 < 0.0);
+        }
+    }
+    return neg;
+}
+
+// D -> |D| (SPD conversion, symspd.c:37-540): |d| for a 1x1 pivot, V |L| V^T
+// for a 2x2 block with eigen-decomposition V L V^T (Jacobi rotation).
+template <typename T>
+static void abs_pivots(IldlLevel<T> &L) {
+    for (int b = 0; b < L.nB; ++b) {
+        const int q = L.dpair.empty() ? -1 : L.dpair[b];
+        if (q < 0) {
+            if (L.d[b] < T(0)) L.d[b] = -L.d[b];
+        } else if (q == b + 1) {
+            const double a = (double)L.d[b], o = (double)L.doff[b], c = (double)L.d[b + 1];
+            const double th = 0.5 * std::atan2(2.0 * o, a - c);
+            const double cs = std::cos(th), sn = std::sin(th);
+            double l1, l2;
+            eig2(a, o, c, l1, l2);
+            const double e1 = std::fabs(l1), e2 = std::fabs(l2);
+            L.d[b] = (T)(cs * cs * e1 + sn * sn * e2);
+            L.d[b + 1] = (T)(sn * sn * e1 + cs * cs * e2);
+            L.doff[b] = L.doff[b + 1] = (T)(cs * sn * (e1 - e2));
+        }
+    }
+}
+
 template <typename T>
 IldlStats ildl_factor(const SymCsr &A0, const IldlParams &prm, IldlMultilevel<T> &out,
                       long nnz_upper_ref) {
@@ -801,7 +1140,7 @@ IldlStats ildl_factor(const SymCsr &A0, const IldlParams &prm, IldlMultilevel<T>
         }
         st.level_n.push_back(cur.n);
         st.level_nB.push_back(lo.L.nB);
-        for (auto v : lo.L.d) nneg += (v < T(0));
+        nneg += negative_pivots(lo.L);
         out.levels.push_back(std::move(lo.L));
         if (lo.nC == 0) {
             cur = SymCsr();
@@ -825,8 +1164,7 @@ IldlStats ildl_factor(const SymCsr &A0, const IldlParams &prm, IldlMultilevel<T>
         nnz_stored += (long)cur.n * (cur.n + 1) / 2;
     }
     if (prm.spd)
-        for (auto &L : out.levels)
-            for (auto &v : L.d) v = (v < T(0)) ? -v : v;
+        for (auto &L : out.levels) abs_pivots(L);
     out.nneg = nneg;
     out.nnz_factor = nnz_stored;
     st.nlev = (int)out.levels.size() + (out.m > 0 ? 1 : 0);

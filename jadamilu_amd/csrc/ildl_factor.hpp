@@ -40,7 +40,11 @@ struct IldlLevel {
     // own rows of the deferred pivots (coupling 1)
     std::vector<int> lc_ptr, lc_idx;
     std::vector<T> lc_val;
-    std::vector<T> d;  // pivots d_k of B (SPD-converted: |d_k|)
+    std::vector<T> d;  // diagonal of the block diagonal D of B (SPD-converted: |D|)
+    // 2x2 pivots: dpair[b] = the other row of b's 2x2 block (adjacent, b +- 1)
+    // or -1 for a 1x1 pivot; doff[b] = the block's off-diagonal entry
+    std::vector<T> doff;
+    std::vector<int> dpair;
 };
 
 template <typename T>
@@ -69,6 +73,7 @@ struct IldlParams {
     bool spd = false;        // convert pivots to |d| (ISEARCH <= 1)
     int ordering = 2;        // 0 natural, 1 reverse Cuthill-McKee, 2 approximate minimum degree
     int coupling = 0;        // coupling block stored per level (IldlMultilevel::coupling)
+    bool matching = true;    // symmetric weighted matching + compressed AMD (ordering 2)
     int threads = 1;         // host threads for the Schur complements
 };
 

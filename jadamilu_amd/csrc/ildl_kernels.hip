@@ -282,6 +282,22 @@ __global__ __launch_bounds__(kB) void k_dscale(int n, const double *__restrict__
 }
 
 template <typename W>
+__global__ __launch_bounds__(kB) void k_dscale2(int n, const double *__restrict__ dinv,
+                                                const double *__restrict__ doff,
+                                                const int *__restrict__ pair, W *x) {
+    for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB) {
+        const int p = pair[i];
+        if (p < 0) {
+            x[i] = (W)((double)x[i] * dinv[i]);
+        } else if (i < p) {
+            const double xi = (double)x[i], xp = (double)x[p];
+            x[i] = (W)(dinv[i] * xi + doff[i] * xp);
+            x[p] = (W)(doff[p] * xi + dinv[p] * xp);
+        }
+    }
+}
+
+template <typename W>
 __global__ __launch_bounds__(kB) void k_zero(int n, W *x) {
     for (long i = blockIdx.x * (long)kB + threadIdx.x; i < n; i += (long)gridDim.x * kB)
         x[i] = W(0);
@@ -410,6 +426,16 @@ void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x) {
     if (n <= 0) return;
     k_dscale<W><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, x);
 }
+template <typename W>
+void ildl_dscale2(hipStream_t s, int n, const double *dinv, const double *doff, const int *pair,
+                  W *x) {
+    if (n <= 0) return;
+    k_dscale2<W><<<blocks_for(n, kB), kB, 0, s>>>(n, dinv, doff, pair, x);
+}
+template void ildl_dscale2<double>(hipStream_t, int, const double *, const double *, const int *,
+                                   double *);
+template void ildl_dscale2<float>(hipStream_t, int, const double *, const double *, const int *,
+                                  float *);
 template <typename W>
 void ildl_zero(hipStream_t s, int n, W *x) {
     if (n <= 0) return;

@@ -74,6 +74,12 @@ void ildl_sub_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr, c
                      const T *val, const W *x, W *y);
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x);
+// x = D^-1 x with 2x2 blocks: row i pairs with pair[i] (-1: 1x1 pivot,
+// x_i *= dinv_i); a pair is updated by its lower index,
+// (x_i, x_p) <- (dinv_i x_i + doff_i x_p, doff_p x_i + dinv_p x_p)
+template <typename W>
+void ildl_dscale2(hipStream_t s, int n, const double *dinv, const double *doff, const int *pair,
+                  W *x);
 // x = 0 (a kernel, not a memset: the apply may be captured in a HIP graph)
 template <typename W>
 void ildl_zero(hipStream_t s, int n, W *x);

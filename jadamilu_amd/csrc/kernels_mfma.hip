@@ -16,7 +16,8 @@
 //
 // Lane layout of v_mfma_f64_16x16x4_f64 on a wave of 64 lanes (A 16 x 4,
 // B 4 x 16, C/D 16 x 16):  lane l holds A(l%16, l/16), B(l/16, l%16) and
-// D(4*(l/16) + r, l%16) in accumulator element r = 0..3.
+// D(4*r + l/16, l%16) in accumulator element r = 0..3 (measured on gfx950
+// with one-hot operands for every lane pair).
 #include "kernels.hpp"
 
 namespace jdhip {
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(kGB) void k_gram(const double *__restrict__ V, long
             const int j = tj * 16 + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = ti * 16 + 4 * lk + r;
+                const int i = ti * 16 + 4 * r + lk;
                 if (i < K && j < M) P[i + (long)j * K] = acc[t][r];
             }
         }
@@ -135,9 +136,9 @@ __global__ __launch_bounds__(kGB) void k_tall_gemm_mfma(const double *__restrict
 #pragma unroll
             for (int s = 0; s < KS; ++s)
                 acc = mfma(a[s], qs[(4 * s + lk) + (tj * 16 + li) * KP], acc);
-            // D(4*lk + r, li) -> ys[col li][row]
+            // D(4*r + lk, li) -> ys[col li][row]
 #pragma unroll
-            for (int r = 0; r < 4; ++r) ys[wave][li * 17 + 4 * lk + r] = acc[r];
+            for (int r = 0; r < 4; ++r) ys[wave][li * 17 + 4 * r + lk] = acc[r];
             __builtin_amdgcn_wave_barrier();
             // 4 columns per pass, 16 contiguous rows each
 #pragma unroll

@@ -36,6 +36,35 @@ class IldlFactor:
         self.n = n
         self.single = single
 
+    def save(self, path):
+        """Write the factor to a file (jdhip_ildl_save)."""
+        rc = self._L.jdhip_ildl_save(self.h, str(path).encode())
+        if rc != 0:
+            raise OSError(f"jdhip_ildl_save({path}): {rc}")
+
+    @classmethod
+    def load(cls, path):
+        """Read a factor written by save() (jdhip_ildl_load)."""
+        L = lib()
+        err = C.c_int(0)
+        h = L.jdhip_ildl_load(str(path).encode(), C.byref(err))
+        if not h:
+            raise OSError(f"jdhip_ildl_load({path}): {err.value}")
+        f = cls.__new__(cls)
+        f._L = L
+        f.h = h
+        f.ierr = 0
+        m = C.c_int()
+        nlev = L.jdhip_ildl_nlevels(h, C.byref(m))
+        if nlev > 0:
+            n, nB, nlb, nlc = C.c_int(), C.c_int(), C.c_long(), C.c_long()
+            L.jdhip_ildl_level_sizes(h, 0, C.byref(n), C.byref(nB), C.byref(nlb), C.byref(nlc))
+            f.n = n.value
+        else:
+            f.n = m.value
+        f.single = None
+        return f
+
     def close(self):
         if self.h:
             self._L.jdhip_ildl_free(self.h)
@@ -79,6 +108,11 @@ class IldlFactor:
                                     _p(lev["lb_val"], C.c_double), _p(lev["lc_ptr"], C.c_int),
                                     _p(lev["lc_idx"], C.c_int), _p(lev["lc_val"], C.c_double),
                                     _p(lev["d"], C.c_double))
+            doff = np.zeros(max(nB, 1))
+            dpair = np.full(max(nB, 1), -1, np.int32)
+            L.jdhip_ildl_level_pivots(self.h, l, _p(lev["d"], C.c_double), _p(doff, C.c_double),
+                                      _p(dpair, C.c_int))
+            lev["doff"], lev["dpair"] = doff[:nB], dpair[:nB]
             lev["lb_idx"], lev["lb_val"] = lev["lb_idx"][:nlb], lev["lb_val"][:nlb]
             lev["lc_idx"], lev["lc_val"] = lev["lc_idx"][:nlc], lev["lc_val"][:nlc]
             lev["d"] = lev["d"][:nB]

@@ -26,7 +26,27 @@ typedef struct {
     const int *lc_ptr, *lc_idx;
     const double *lc_val;
     const double *d;
+    const double *doff; /* 2x2 pivots: off-diagonal entry of b's block */
+    const int *dpair;   /* the other row of b's block, -1 for 1x1 (NULL: all 1x1) */
 } jdo_ildl_level;
+
+/* x_B = D^-1 x_B, D block diagonal with 1x1 and 2x2 blocks (symAMGsol.c:
+ * 221-300) */
+static void dsolve(const jdo_ildl_level *V, double *x)
+{
+    for (int b = 0; b < V->nB; b++) {
+        const int p = V->dpair ? V->dpair[b] : -1;
+        if (p < 0) {
+            x[b] /= V->d[b];
+        } else if (b < p) {
+            const double a = V->d[b], o = V->doff[b], c = V->d[p];
+            const double det = a * c - o * o;
+            const double xb = x[b], xp = x[p];
+            x[b] = (c * xb - o * xp) / det;
+            x[p] = (a * xp - o * xb) / det;
+        }
+    }
+}
 
 /* 0: exact triangular solves; k > 0: each triangular solve replaced by k
  * Jacobi sweeps x <- b - N x from x = b (N the strictly triangular part),
@@ -72,7 +92,7 @@ static void solve_b(const jdo_ildl_level *V, double *x)
         for (int k = V->lb_ptr[i]; k < V->lb_ptr[i + 1]; k++) s += V->lb_val[k] * x[V->lb_idx[k]];
         x[i] -= s;
     }
-    for (int b = 0; b < nB; b++) x[b] /= V->d[b];
+    dsolve(V, x);
     if (g_sweeps > 0) jacobi_lower(nB, V->lb_ptr, V->lb_idx, V->lb_val, x, 1);
     else for (int i = nB - 1; i >= 0; i--) {
         double xi = x[i];
@@ -158,7 +178,7 @@ static void level_apply(int l, int nlev, const jdo_ildl_level *L, int m, const d
         for (int k = V->lc_ptr[c]; k < V->lc_ptr[c + 1]; k++) s += V->lc_val[k] * buf[V->lc_idx[k]];
         buf[nB + c] -= s;
     }
-    for (int b = 0; b < nB; b++) buf[b] /= V->d[b];
+    dsolve(V, buf);
     if (nC > 0) level_apply(l + 1, nlev, L, m, minv, buf + nB, zc);
     /* B rows minus L_CB^T z_C */
     for (int c = 0; c < nC; c++)

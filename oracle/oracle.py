@@ -199,7 +199,8 @@ def matvec_upper(ia, ja, a):
 class _Level(C.Structure):
     _fields_ = [("n", C.c_int), ("nB", C.c_int), ("perm", _i), ("scal", _d),
                 ("lb_ptr", _i), ("lb_idx", _i), ("lb_val", _d),
-                ("lc_ptr", _i), ("lc_idx", _i), ("lc_val", _d), ("d", _d)]
+                ("lc_ptr", _i), ("lc_idx", _i), ("lc_val", _d), ("d", _d),
+                ("doff", _d), ("dpair", _i)]
 
 
 class _IldlCtx(C.Structure):
@@ -212,13 +213,14 @@ def _levels_struct(factor):
     keep = [arr]
     for k, lv in enumerate(levs):
         fields = {}
-        for name in ("perm", "lb_ptr", "lb_idx", "lc_ptr", "lc_idx"):
-            a = np.ascontiguousarray(lv[name], dtype=np.int32)
+        for name in ("perm", "lb_ptr", "lb_idx", "lc_ptr", "lc_idx", "dpair"):
+            a = np.ascontiguousarray(lv.get(name, np.full(max(lv["nB"], 1), -1)),
+                                     dtype=np.int32)
             if a.size == 0:
                 a = np.zeros(1, np.int32)
             keep.append(a)
             fields[name] = a.ctypes.data_as(_i)
-        for name in ("scal", "lb_val", "lc_val", "d"):
+        for name in ("scal", "lb_val", "lc_val", "d", "doff"):
             a = np.ascontiguousarray(lv[name], dtype=np.float64)
             if a.size == 0:
                 a = np.zeros(1)
@@ -283,12 +285,15 @@ def ildl_apply(factor, r, sweeps=0):
     keep = []
     for k, lv in enumerate(levs):
         fields = {}
-        for name in ("perm", "lb_ptr", "lb_idx", "lc_ptr", "lc_idx"):
-            a = np.ascontiguousarray(lv[name], dtype=np.int32)
+        for name in ("perm", "lb_ptr", "lb_idx", "lc_ptr", "lc_idx", "dpair"):
+            a = np.ascontiguousarray(lv.get(name, np.full(max(lv["nB"], 1), -1)),
+                                     dtype=np.int32)
+            if a.size == 0:
+                a = np.zeros(1, np.int32)
             keep.append(a)
             fields[name] = a.ctypes.data_as(_i)
-        for name in ("scal", "lb_val", "lc_val", "d"):
-            a = np.ascontiguousarray(lv[name], dtype=np.float64)
+        for name in ("scal", "lb_val", "lc_val", "d", "doff"):
+            a = np.ascontiguousarray(lv.get(name, np.zeros(max(lv["nB"], 1))), dtype=np.float64)
             if a.size == 0:
                 a = np.zeros(1)
             keep.append(a)

@@ -263,3 +263,40 @@ def test_tuning_without_accurate_sweeps_keeps_exact_solves():
     lv = ctx.ildl_level_sweeps()
     assert len(lv) >= 1 and all(k == 0 for k in lv), lv
     ctx.close()
+
+
+def test_saved_factor_reused_in_a_solve(tmp_path):
+    """A factor written to disk and read back (jdhip_ildl_save / _load),
+    attached to a fresh context and used with ICNTL(2) = -1, gives exactly the
+    eigenvalues and MATVEC count of the same factor attached without the file
+    round trip (kernel forms and triangular solves fixed, so both runs
+    execute the same launches)."""
+    from jadamilu_amd.ildl import IldlFactor
+    from jadamilu_amd.jd import Context, lx_optimal
+    dev = _dev()
+    shape = (16, 15, 14)
+    ia, ja, a = M.laplace(shape)
+    n = ia.size - 1
+    f = IldlFactor(n, ia, ja, a, shift=0.0, droptol=1e-3, spd=True)
+    path = tmp_path / "lap.jdf"
+    f.save(path)
+    g = IldlFactor.load(path)
+    out = []
+    for fac in (f, g):
+        ctx = Context(0)
+        ctx.set_option("ildl_autotune", 0)
+        ctx.attach_ildl(fac, sweeps=0)
+        lx = lx_optimal(n, 6, 20)
+        X = torch.zeros(lx, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        r = ctx.pjd(n, a, ja, ia, X.data_ptr(), lx, neig=6, isearch=1, sigma=0.0, madspace=20,
+                    maxiter=4000, tol=1e-10, droptol=1e-3, mem=20.0, icntl=(0, -1, 0, 0, 0))
+        torch.cuda.synchronize()
+        assert r.info == 0 and r.neig == 6
+        assert r.stats["n_precond_setups"] == 0
+        out.append(r)
+        ctx.close()
+    assert out[0].iter == out[1].iter
+    assert np.array_equal(out[0].eigs, out[1].eigs)
+    lam, _ = M.laplace_eigs(shape, 6)
+    np.testing.assert_allclose(np.sort(out[1].eigs), lam, rtol=1e-10)

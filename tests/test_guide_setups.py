@@ -41,10 +41,7 @@ _ISEARCH = {"EXAMPLE1": 2, "EXAMPLE2": 0, "EXAMPLE11": 0, "LISTING5_ADAPTIVE": 0
 
 # Set-ups whose fill the host factor does not reproduce yet (strict=False:
 # an xpass is reported, not failed)
-_OPEN = {
-    "EXAMPLE1": "indefinite tridiagonal: the reference pairs rows into 2x2 pivots from a "
-                "maximum-product matching (sympermMC64amd.c, symwm.c); fill differs",
-}
+_OPEN = {}
 
 
 def _case(name, k):

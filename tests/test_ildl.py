@@ -167,3 +167,39 @@ def test_oracle_sweep_mode_is_exact_beyond_the_depth(gen, spd):
     np.testing.assert_allclose(zk, z0, rtol=0, atol=1e-12 * np.abs(z0).max())
     z1 = O.ildl_apply(ex, r, sweeps=1)
     assert np.linalg.norm(z1 - z0) > 1e-6 * np.linalg.norm(z0)
+
+
+def test_two_by_two_pivots_from_the_matching():
+    """On indefinite matrices the symmetric weighted matching pairs rows into
+    2x2 pivots (symwm.c, sympiluc.F:1042-1200): the blocks are adjacent
+    (dpair[b] = b +- 1, mutual), nonsingular, counted into the inertia, and
+    |D| of every 2x2 block is positive definite under the SPD conversion.
+    The exact (droptol 0) factor with them reproduces A (M^-1 A x = x)."""
+    ia, ja, a = M.anderson(8, 16.5, 6)
+    n = ia.size - 1
+    err, f, ex = _relerr(ia, ja, a, 0.0, droptol=0.0, dense_max=20)
+    assert err < 1e-11, err
+    npairs = 0
+    for lv in ex["levels"]:
+        dp, d, o = lv["dpair"], lv["d"], lv["doff"]
+        for b in range(lv["nB"]):
+            q = dp[b]
+            if q < 0:
+                assert o[b] == 0.0
+                continue
+            assert abs(q - b) == 1 and dp[q] == b and o[q] == o[b]
+            if q == b + 1:
+                npairs += 1
+                assert d[b] * d[q] - o[b] ** 2 != 0.0
+    assert npairs > 0
+    w = np.linalg.eigvalsh(M.to_dense(ia, ja, a))
+    assert f.stats()["nneg"] == int((w < 0).sum())
+    g = ildl.IldlFactor(n, ia, ja, a, shift=0.0, droptol=1e-3, spd=True, dense_max=20)
+    for lv in g.export()["levels"]:
+        dp, d, o = lv["dpair"], lv["d"], lv["doff"]
+        for b in range(lv["nB"]):
+            if dp[b] < 0:
+                assert d[b] > 0
+            elif dp[b] == b + 1:
+                q = b + 1
+                assert d[b] > 0 and d[q] > 0 and d[b] * d[q] - o[b] ** 2 > 0
