@@ -130,10 +130,11 @@ typedef struct {
 /* Solve every item on HIP device `device` with nworkers worker threads, each
  * owning a context and a HIP stream; items are taken largest first.  Each
  * item starts from the state of a fresh process and makes its set-up choices
- * from its inputs alone (options ildl_autotune = 0, ildl_tune_time = 0), so
- * its results equal those of a single jdhip_pjd call with these options.  No
- * reference counterpart (the reference solves one problem per call).  Returns the number of items with INFO != 0, or -1
- * on invalid arguments. */
+ * from its inputs alone (options ildl_autotune = 0, ildl_tune_time = 0,
+ * cost_clock = 1), so its results equal those of a single jdhip_pjd call with
+ * these options.  No reference counterpart (the reference solves one problem
+ * per call).  Returns the number of items with INFO != 0, or -1 on invalid
+ * arguments. */
 int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers);
 
 /* ---- symmetric SpMV (MATVECA, JDaux.F:3397-3418) ------------------------- */
@@ -190,7 +191,12 @@ int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, l
  * counts are kept only when timed faster than exact solves, the default; 0 =
  * whenever accurate), "fail_setup" (k > 0: the k-th preconditioner set-up of
  * a call reports failure, to exercise the INFO = -54321 path of
- * JDaux.F:431-441; default 0).  0 on success, -2 for an
+ * JDaux.F:431-441; default 0), "cost_clock" (clock of PJDRVCOM's
+ * re-factorisation cost model, JDaux.F:282-449 and 505-535: 0 = wall clock,
+ * the default; 1 = every interval reads 0 s, so the reference's floors
+ * decide, as on a machine where the problem is small: the set-up sequence and
+ * the suggested DROPTOL then follow from the residual history alone).
+ * 0 on success, -2 for an
  * unknown name, -3 for an invalid value. */
 int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value);
 

@@ -9,13 +9,15 @@
 // items run in parallel on the CPU cores and the device work of different
 // items overlaps on the GPU through the workers' streams, which keeps the GPU
 // busy even when a single small problem cannot fill it.  Every item starts
-// from the state of a fresh process (jdhip_reset).  The set-up choices that
-// a single jdhip_pjd call makes by timing (kernel form of each factor
-// product, sweeps against exact solves) are made here from the inputs alone
-// (forms by row length, sweep counts by accuracy), because timings taken
-// while other workers share the GPU are not reproducible: an item's result
-// therefore depends only on its inputs, and equals that of a jdhip_pjd call
-// with the options ildl_autotune = 0 and ildl_tune_time = 0.
+// from the state of a fresh process (jdhip_reset).  The choices that a
+// single jdhip_pjd call makes by timing (kernel form of each factor product,
+// sweeps against exact solves, re-factorisation by PJDRVCOM's cost model) are
+// made here from the inputs alone (forms by row length, sweep counts by
+// accuracy, the cost model with every interval read as 0 s), because timings
+// taken while other workers share the GPU are not reproducible: an item's
+// result therefore depends only on its inputs, and equals that of a jdhip_pjd
+// call with the options ildl_autotune = 0, ildl_tune_time = 0 and
+// cost_clock = 1.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,6 +58,7 @@ extern "C" int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, 
         // set-up choices from the inputs only (see the header comment)
         jdhip_set_option(ctx, "ildl_autotune", 0);
         jdhip_set_option(ctx, "ildl_tune_time", 0);
+        jdhip_set_option(ctx, "cost_clock", 1);
         double *x = nullptr;
         long xcap = 0;
         std::vector<int> ia, ja;

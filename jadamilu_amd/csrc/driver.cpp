@@ -84,6 +84,12 @@ struct Solver {
     // set-up number (1-based, counted from the start of a call) that is made
     // to fail, so that callers can exercise the INFO = -54321 path; 0: none
     int fail_setup = 0;
+    // clock of the re-factorisation cost model (JDaux.F:282-449, 505-535):
+    // 0 wall clock; 1 every interval reads 0 s, so that the reference's
+    // floors decide (solve time < 0.1 s counts as 0, factor and final solve
+    // times count as 0.01 s): the set-up sequence of a machine on which the
+    // problem is small, independent of this machine's speed
+    int cost_clock = 0;
     IldlResult ildl_info;
     bool reenterfirsttime = false, flagsingle = false;
     int nw = 0, ierr = 0;
@@ -170,7 +176,8 @@ struct Solver {
     double *gap_ = nullptr;
     double gap_dummy_ = 0.0;
     std::chrono::steady_clock::time_point tic_ = std::chrono::steady_clock::now();
-    static double seconds_since(std::chrono::steady_clock::time_point t) {
+    double seconds_since(std::chrono::steady_clock::time_point t) const {
+        if (cost_clock == 1) return 0.0;
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
     }
     void dglprecsetup(int isearch_, double shift_in, double *shift0_out, double *diagmin_out);
@@ -543,7 +550,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         totaltime =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         precond_seconds += totaltime;
-        timefact = std::max(totaltime, 1.0e-2);
+        timefact = std::max(cost_clock == 1 ? 0.0 : totaltime, 1.0e-2);
         factdgl = -1;
         droptol0 = *droptol;
         condest0 = r.condest;
@@ -1010,6 +1017,10 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_autotune") S.ildl_autotune = value != 0.0;
     else if (k == "ildl_tune_tol") S.ildl_tune_tol = value;
     else if (k == "fail_setup") S.fail_setup = (int)value;
+    else if (k == "cost_clock") {
+        if (value != 0.0 && value != 1.0) return -3;
+        S.cost_clock = (int)value;
+    }
     else if (k == "ildl_tune_time") S.ildl_tune_time = value != 0.0;
     else if (k == "ildl_form") {
         const int f = (int)value;
