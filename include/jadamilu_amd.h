@@ -162,6 +162,17 @@ int jdhip_spmm(jdhip_ctx *ctx, int k, const double *x_dev, long ldx, double *y_d
  * scratch.  0 on success, -1 bad arguments, -2 HIP error. */
 int jdhip_gram(jdhip_ctx *ctx, long n, int k, int m, const double *v_dev, long ldv,
                const double *w_dev, long ldw, double *c_dev, int ldc);
+/* jdhip_gram without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * bit-identical result as jdhip_gram, on `stream` (a hipStream_t of the
+ * current device), with no context: the partial sums live in a static
+ * device array of 4 MiB (524,288 doubles) that each GPU holds until the
+ * process ends; k, m <= 64 (larger sizes return -1 with a message on
+ * stderr).  Calls that overlap in time on one GPU share that array and race;
+ * serialise them.  0 on success, -1 bad arguments, -2 HIP error. */
+int jdhip_gram_noalloc(void *stream, long n, int k, int m, const double *v_dev, long ldv,
+                       const double *w_dev, long ldw, double *c_dev, int ldc);
 /* Y = V Q (n x j) for V (n x k), Q (k x j), k, j <= 64, DEVICE, column-major,
  * on the matrix cores: the basis update of SHRINK (JDaux.F:2418-2462). */
 int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, long ldv,

@@ -78,6 +78,8 @@ SIGNATURES = {
     "jdhip_spmm": (C.c_int, [_v, C.c_int, _v, C.c_long, _v, C.c_long]),
     "jdhip_gram": (C.c_int, [_v, C.c_long, C.c_int, C.c_int, _v, C.c_long, _v, C.c_long, _v,
                              C.c_int]),
+    "jdhip_gram_noalloc": (C.c_int, [_v, C.c_long, C.c_int, C.c_int, _v, C.c_long, _v, C.c_long,
+                                     _v, C.c_int]),
     "jdhip_tall_gemm": (C.c_int, [_v, C.c_long, C.c_int, C.c_int, _v, C.c_long, _v, C.c_int,
                                   _v, C.c_long]),
     "jdhip_get_stats": (None, [_v, C.POINTER(JdhipStats)]),

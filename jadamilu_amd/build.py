@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libjadamilu_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JDHIP_ARCH", "gfx950")
 
-SOURCES = ["kernels_blas.hip", "kernels_inner.hip", "kernels_mfma.hip", "kernels_spmm.hip", "ildl_kernels.hip", "engine.cpp",
+SOURCES = ["kernels_blas.hip", "kernels_inner.hip", "kernels_mfma.hip", "kernels_spmm.hip", "noalloc.hip", "ildl_kernels.hip", "engine.cpp",
            "dense.cpp", "driver.cpp", "ildl.cpp", "ildl_factor.cpp", "ordering.cpp", "matching.cpp", "batch.cpp", "persist.cpp",
            "adjoint.hip"]
 
