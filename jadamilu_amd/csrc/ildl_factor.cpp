@@ -190,16 +190,16 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     // column, jw(ne+1:ne+n))
     std::vector<double> est1(n, 0.0), est2(n, 0.0);
     std::vector<int> peaks(n, 0);
-    // column arena (accepted columns, rows sorted)
-    std::vector<long> cbeg(n, 0), cend(n, 0), cur(n, 0);
+    // column arena (accepted columns, rows sorted).  Column p occupies
+    // [cbeg, cend): its entries of deferred rows above the current row are
+    // compacted into [cbeg, dend), its entries of rows not yet reached are
+    // [cur, cend), and the slots between hold entries of accepted rows,
+    // which no later column needs (the level output reads the row lists)
+    std::vector<long> cbeg(n, 0), cend(n, 0), cur(n, 0), dend(n, 0);
     std::vector<int> crow;
     std::vector<T> cval;
     crow.reserve(ai.size() * 2);
     cval.reserve(ai.size() * 2);
-    // deferred-row entries per column (linked lists)
-    std::vector<long> dhead(n, -1), dnext;
-    std::vector<int> drow;
-    std::vector<T> dval;
     // row lists of L: rows -> (column, value)
     std::vector<long> rhead(n, -1), rtail(n, -1), rnext;
     std::vector<int> rcol;
@@ -207,72 +207,74 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     rnext.reserve(ai.size() * 2);
     rcol.reserve(ai.size() * 2);
     rval.reserve(ai.size() * 2);
-    // sparse accumulator
-    std::vector<T> wv(n, T(0));
-    std::vector<int> wmark(n, -1), wlist;
+    // sparse accumulators (branch-free insertion: the index is always
+    // written at the end of the list and the list grows only on first touch)
+    struct Acc {
+        std::vector<T> v;
+        std::vector<int> mark, list;
+        int len = 0;
+        explicit Acc(int n_) : v(n_, T(0)), mark(n_, -1), list(n_ + 1) {}
+        void clear() { len = 0; }
+        void add(int i, T x, int k) {
+            const bool seen = mark[i] == k;
+            v[i] = seen ? v[i] + x : x;
+            list[len] = i;
+            len += seen ? 0 : 1;
+            mark[i] = k;
+        }
+        const int *begin() const { return list.data(); }
+        const int *end() const { return list.data() + len; }
+    };
+    Acc A1(n), A2(n);  // column k, and column k+1 of a 2x2 pivot candidate
+    std::vector<T> &wv = A1.v, &wv2 = A2.v;
+    std::vector<int> &wmark = A1.mark, &wmark2 = A2.mark;
+    const Acc &wlist = A1, &wlist2 = A2;
     std::vector<std::pair<int, T>> kept;
     int nB = 0, nC = 0;
     double maxcond = 1.0;
-    // second accumulator: column k+1 of a 2x2 pivot candidate
-    std::vector<T> wv2(n, T(0));
-    std::vector<int> wmark2(n, -1), wlist2;
     // 2x2 pivots: off-diagonal entry of the block and the partner row
     std::vector<T> doff(n, T(0));
     std::vector<int> partner(n, -1);
-    auto acc1 = [&](int i, T v, int k) {
-        if (wmark[i] != k) {
-            wmark[i] = k;
-            wv[i] = v;
-            wlist.push_back(i);
-        } else {
-            wv[i] += v;
-        }
-    };
-    auto acc2 = [&](int i, T v, int k) {
-        if (wmark2[i] != k) {
-            wmark2[i] = k;
-            wv2[i] = v;
-            wlist2.push_back(i);
-        } else {
-            wv2[i] += v;
-        }
-    };
     // Column j of the not yet factored part: A(:,j) minus the updates of the
     // accepted pivots (1x1, or 2x2 blocks D_pq), into accumulator 1 or 2.
     // advance: move the column pointers past the rows before j (the deferred
     // ones go to the column's deferred list); the 2x2 partner column is
     // gathered without advancing.  Returns the updated diagonal entry.
     auto gather = [&](int j, bool second) -> T {
-        auto acc = [&](int i, T v) {
-            if (second) acc2(i, v, j);
-            else acc1(i, v, j);
-        };
+        Acc &W = second ? A2 : A1;
         T wj = T(0);
         for (int t = ap[j]; t < ap[j + 1]; ++t) {
             const int i = ai[t];
             if (i == j) wj = av[t];
-            else if (status[i] != 1) acc(i, av[t]);
+            else if (status[i] != 1) W.add(i, av[t], j);
         }
         auto apply_col = [&](int p, T coef, T ljp) {
             wj -= coef * ljp;
             if (!second) {
+                // pass the rows above j: deferred ones move to the compacted
+                // prefix, accepted ones are dropped from the column
                 while (cur[p] < cend[p] && crow[cur[p]] < j) {
-                    const int r = crow[cur[p]];
-                    if (status[r] == 2) {
-                        dnext.push_back(dhead[p]);
-                        drow.push_back(r);
-                        dval.push_back(cval[cur[p]]);
-                        dhead[p] = (long)drow.size() - 1;
+                    const long t = cur[p]++;
+                    if (status[crow[t]] == 2) {
+                        crow[dend[p]] = crow[t];
+                        cval[dend[p]] = cval[t];
+                        ++dend[p];
                     }
-                    cur[p]++;
+                }
+                // rows >= j are all pending here
+                for (long t = cur[p]; t < cend[p]; ++t) {
+                    const int r = crow[t];
+                    if (r != j) W.add(r, -coef * cval[t], j);
+                }
+            } else {
+                // column k+1 of a 2x2 candidate: the pointers stay at row k
+                for (long t = cur[p]; t < cend[p]; ++t) {
+                    const int r = crow[t];
+                    if (r != j && status[r] != 1) W.add(r, -coef * cval[t], j);
                 }
             }
-            for (long t = cur[p]; t < cend[p]; ++t) {
-                const int r = crow[t];
-                if (r != j && status[r] != 1) acc(r, -coef * cval[t]);
-            }
-            for (long t = dhead[p]; t >= 0; t = dnext[t])
-                if (drow[t] != j) acc(drow[t], -coef * dval[t]);
+            // deferred rows (all above j), latest first
+            for (long t = dend[p] - 1; t >= cbeg[p]; --t) W.add(crow[t], -coef * cval[t], j);
         };
         for (long e = rhead[j]; e >= 0; e = rnext[e]) {
             const int p = rcol[e];
@@ -313,12 +315,12 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             rtail[kv.first] = id;
         }
         cend[k] = (long)crow.size();
-        cur[k] = cbeg[k];
+        cur[k] = dend[k] = cbeg[k];
     };
     std::vector<std::pair<int, T>> kept2;
     std::vector<int> urows;
     for (int k = 0; k < n;) {
-        wlist.clear();
+        A1.clear();
         const T wk = gather(k, false);
         // ---- 1x1 or 2x2 pivot (sympiluc.F:1042-1200) ----
         // 2x2 blocks are tried where the matching paired rows k, k+1; the
@@ -332,7 +334,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         if (k + 1 < n && pair_start[k] && status[k + 1] == 0) {
             b = (wmark[k + 1] == k) ? wv[k + 1] : T(0);
             if (b != T(0)) {
-                wlist2.clear();
+                A2.clear();
                 wk1 = gather(k + 1, true);
                 const double A11 = (double)wk, A12 = (double)b, A22 = (double)wk1;
                 const double det = A11 * A22 - A12 * A12;
@@ -650,11 +652,14 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             L.doff[bpos[k]] = doff[k];
             L.dpair[bpos[k]] = bpos[partner[k]];
         }
-        for (long t = cbeg[k]; t < cend[k]; ++t) {
-            int r = crow[t];
-            if (status[r] == 1) lbcnt[bpos[r] + 1]++;
-            else lccnt[cpos[r] + 1]++;
-        }
+    }
+    // rows of L_B and L_CB from the row lists, whose entries are in column
+    // (acceptance) order, so column indices ascend within each row
+    for (int r = 0; r < n; ++r) {
+        int c = 0;
+        for (long e = rhead[r]; e >= 0; e = rnext[e]) ++c;
+        if (status[r] == 1) lbcnt[bpos[r] + 1] = c;
+        else lccnt[cpos[r] + 1] = c;
     }
     for (int i = 0; i < nB; ++i) lbcnt[i + 1] += lbcnt[i];
     for (int i = 0; i < nC; ++i) lccnt[i + 1] += lccnt[i];
@@ -664,31 +669,22 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     L.lb_val.assign(lbcnt[nB], T(0));
     L.lc_idx.assign(lccnt[nC], 0);
     L.lc_val.assign(lccnt[nC], T(0));
-    std::vector<int> fb(lbcnt.begin(), lbcnt.end() - 1), fc(lccnt.begin(), lccnt.end() - 1);
-    // columns visited in acceptance order -> column indices ascending per row
-    std::vector<int> acc_nodes;
-    acc_nodes.reserve(nB);
-    for (int k = 0; k < n; ++k)
-        if (status[k] == 1) acc_nodes.push_back(k);
     // L_CB by columns for the Schur complement
     std::vector<int> lcc_ptr(nB + 1, 0);
-    for (int k : acc_nodes) {
-        int b = bpos[k];
-        int cc = 0;
-        for (long t = cbeg[k]; t < cend[k]; ++t) {
-            int r = crow[t];
-            if (status[r] == 1) {
-                int rr = bpos[r];
-                L.lb_idx[fb[rr]] = b;
-                L.lb_val[fb[rr]++] = cval[t];
+    for (int r = 0; r < n; ++r) {
+        const bool inB = status[r] == 1;
+        int f = inB ? lbcnt[bpos[r]] : lccnt[cpos[r]];
+        for (long e = rhead[r]; e >= 0; e = rnext[e], ++f) {
+            const int b = bpos[rcol[e]];
+            if (inB) {
+                L.lb_idx[f] = b;
+                L.lb_val[f] = rval[e];
             } else {
-                int rr = cpos[r];
-                L.lc_idx[fc[rr]] = b;
-                L.lc_val[fc[rr]++] = cval[t];
-                ++cc;
+                L.lc_idx[f] = b;
+                L.lc_val[f] = rval[e];
+                lcc_ptr[b + 1]++;
             }
         }
-        lcc_ptr[b + 1] = cc;
     }
     if (nC == 0) return;
     for (int b = 0; b < nB; ++b) lcc_ptr[b + 1] += lcc_ptr[b];
@@ -718,26 +714,24 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
         std::vector<int> cnt, idx;
         std::vector<double> val;
     };
-    const int nth = std::max(1, std::min(prm.threads, std::max(1, nC / 4096)));
+    const int nth = std::max(1, std::min(prm.threads, std::max(1, nC / 256)));
     std::vector<Piece> pieces(nth);
     auto schur_rows = [&](int tid) {
         const int c0 = (int)((long)nC * tid / nth), c1 = (int)((long)nC * (tid + 1) / nth);
         Piece &P = pieces[tid];
         P.cnt.assign(c1 - c0, 0);
         std::vector<double> sv(nC, 0.0);
-        std::vector<int> smark(nC, -1), slist;
+        std::vector<int> smark(nC, -1), slist(nC + 1);
         std::vector<std::pair<int, double>> srow;
         for (int c = c0; c < c1; ++c) {
-            slist.clear();
+            int slen = 0;
             int k = cnode[c];
             auto sacc = [&](int j, double v) {
-                if (smark[j] != c) {
-                    smark[j] = c;
-                    sv[j] = v;
-                    slist.push_back(j);
-                } else {
-                    sv[j] += v;
-                }
+                const bool seen = smark[j] == c;
+                sv[j] = seen ? sv[j] + v : v;
+                slist[slen] = j;
+                slen += seen ? 0 : 1;
+                smark[j] = c;
             };
             for (int t = ap[k]; t < ap[k + 1]; ++t) {
                 int j = ai[t];
@@ -773,8 +767,10 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
                 }
             }
             srow.clear();
-            for (int j : slist)
+            for (int u = 0; u < slen; ++u) {
+                const int j = slist[u];
                 if (j == c || std::fabs(sv[j]) > sdrop) srow.emplace_back(j, sv[j]);
+            }
             std::sort(srow.begin(), srow.end(),
                       [](const auto &x1, const auto &y1) { return x1.first < y1.first; });
             P.cnt[c - c0] = (int)srow.size();
