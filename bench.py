@@ -290,6 +290,85 @@ def bench_adjoint(args, params, neig, madspace, isearch, sigma, mem):
     print(json.dumps(out))
 
 
+def bench_c3_pair(args, torch):
+    """FP64 against the mixed path on the Anderson workload of the reference's
+    anderson_mixed sample (L = 100, W = 16.5, 10 eigenpairs nearest 0,
+    ISEARCH = 2, MADSPACE = 25, DROPTOL 1e-3, MEM = 15): per path one set-up
+    solve (host factor + solve) and one timed solve with the factor resident
+    (ICNTL(2) = -1), from the same start vector.  Mixed: FP32 factor
+    (ICNTL(5) = 1) with the FP64 inner solver (the reference's ICNTL(5)), and
+    the same factor with the FP32 inner solver (option inner_fp32)."""
+    from jadamilu_amd.jd import Context, lx_optimal
+    ia, ja, a = build_matrix("anderson", (100, 16.5, 0))
+    n = ia.size - 1
+    neig, madspace, mem = 10, 25, 15.0
+    x0 = torch.tensor(np.random.default_rng(0).random(n), device="cuda:0")
+    lx = lx_optimal(n, neig, madspace)
+    X = torch.zeros(lx, dtype=torch.float64, device="cuda:0")
+    anorm = norm1(ia, ja, a)
+
+    def solve(ctx, icntl2, single):
+        X[:n].copy_(x0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = ctx.pjd(n, a, ja, ia, X.data_ptr(), lx, neig=neig, isearch=2, sigma=0.0, ninit=1,
+                    madspace=madspace, maxiter=args.maxiter, tol=args.tol, droptol=1e-3,
+                    mem=mem, icntl=(0, icntl2, 0, 0, single))
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t
+
+    def entry(r, sec, r_setup, sec_setup):
+        return {"eigenpairs_per_s": (r.neig / sec) if r.info == 0 else 0.0, "seconds": sec,
+                "matvecs": int(r.iter), "neig": int(r.neig), "info": int(r.info),
+                "max_rel_res": (float(np.max(r.res)) / anorm) if r.neig else None,
+                "set_up_solve_s": sec_setup,
+                "host_factor_s": r_setup.stats.get("precond_seconds"),
+                "fill": r_setup.stats.get("fill")}
+
+    out = {"workload": "anderson_100 (L=100, W=16.5, seed 0), MEM=15, DROPTOL=1e-3, "
+                       "NEIG=10, ISEARCH=2, SIGMA=0, MADSPACE=25", "target_speedup": 1.5}
+    ctx = Context(0)
+    try:
+        rs, ss = solve(ctx, 1, 0)
+        print(f"# C3 FP64 set-up solve: {ss:.1f} s, {rs.iter} MATVECs, info {rs.info}",
+              file=sys.stderr, flush=True)
+        r64, s64 = solve(ctx, -1, 0)
+        print(f"# C3 FP64 timed solve: {s64:.1f} s, {r64.iter} MATVECs, info {r64.info}",
+              file=sys.stderr, flush=True)
+        out["fp64"] = entry(r64, s64, rs, ss)
+    finally:
+        ctx.close()
+    ctx = Context(0)
+    try:
+        rs, ss = solve(ctx, 1, 1)
+        print(f"# C3 mixed set-up solve: {ss:.1f} s, {rs.iter} MATVECs, info {rs.info}",
+              file=sys.stderr, flush=True)
+        rm, sm = solve(ctx, -1, 1)
+        print(f"# C3 mixed timed solve: {sm:.1f} s, {rm.iter} MATVECs, info {rm.info}",
+              file=sys.stderr, flush=True)
+        out["mixed_factor32"] = entry(rm, sm, rs, ss)
+        ctx.set_option("inner_fp32", 1)
+        ri, si = solve(ctx, -1, 1)
+        print(f"# C3 mixed (FP32 inner) timed solve: {si:.1f} s, {ri.iter} MATVECs, "
+              f"info {ri.info}", file=sys.stderr, flush=True)
+        out["mixed_inner32"] = entry(ri, si, rs, ss)
+    finally:
+        ctx.close()
+    e64 = np.asarray(r64.eigs)
+    for key, rr in (("mixed_factor32", rm), ("mixed_inner32", ri)):
+        o = out[key]
+        o["speedup_vs_fp64"] = (o["eigenpairs_per_s"] / out["fp64"]["eigenpairs_per_s"]
+                                if out["fp64"]["eigenpairs_per_s"] > 0 else None)
+        o["meets_target"] = bool(o["speedup_vs_fp64"] and o["speedup_vs_fp64"] >= 1.5)
+        if rr.neig and r64.neig:
+            near = [float(e64[np.argmin(np.abs(e64 - l))]) for l in rr.eigs]
+            rel = [abs(l - m) / max(abs(m), 1e-300) for l, m in zip(rr.eigs, near)]
+            o["agree_with_fp64_1e-10"] = int(sum(1 for x in rel if x <= 1e-10))
+            o["max_abs_err_over_norm1"] = float(max(abs(l - m) for l, m in
+                                                    zip(rr.eigs, near))) / anorm
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +398,8 @@ def main():
                     help="extra jdhip_set_option(NAME, VALUE), repeatable")
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts of the batched workload")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the FP64-against-mixed block on the anderson_100 workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV and the first-level ILDL sweep of the workload (and a "
                          "plain device copy of known size for calibrating HBM counters), "
@@ -567,6 +648,11 @@ def main():
                           f"value = solve with the factor resident (as the GPU value), "
                           f"end_to_end_value adds the host factorisation ({fac_s:.1f} s)")}
 
+    ctx.close()
+    c3 = None
+    if not args.no_c3 and args.workload == "laplace3d_128":
+        c3 = bench_c3_pair(args, torch)
+
     out = {
         "metric": "eigenpairs/s to rel. residual 1e-10 (mixed vs FP64 GPU vs CPU); SpMV GB/s",
         "value": eig_per_s,
@@ -612,9 +698,9 @@ def main():
                    "fill": r.stats.get("fill"), "precond_levels": r.stats.get("precond_levels"),
                    "vs_fp64": vs64,
                    "ildl_device": pinfo},
+        "mixed_vs_fp64": c3,
     }
     print(json.dumps(out))
-    ctx.close()
 
 
 if __name__ == "__main__":
