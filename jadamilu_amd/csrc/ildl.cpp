@@ -464,11 +464,11 @@ class IldlImpl : public IldlImplBase {
         if (form == 1) {
             ildl_jacobi_stream<T, W>(s, fwd ? D.nrb_b : D.nrb_t, fwd ? D.rb_b : D.rb_t,
                                      fwd ? D.lb_ptr : D.lt_ptr, fwd ? D.lb_idx : D.lt_idx,
-                                     fwd ? D.lb_val : D.lt_val, b, xin, xout);
+                                     fwd ? D.lb_val : D.lt_val, b, xin, xout, xcd_ & 1);
         } else if (form > 0) {
             ildl_jacobi_csr<T, W>(s, form, D.nB, fwd ? D.lb_ptr : D.lt_ptr,
                                   fwd ? D.lb_idx : D.lt_idx, fwd ? D.lb_val : D.lt_val, b, xin,
-                                  xout);
+                                  xout, xcd_ & 1);
         } else {
             const SellDev<T> &M = fwd ? D.sf : D.sb;
             ildl_jacobi_sweep_sell<T, W>(s, M.v, M.val, b, xin, xout, xcd_);
@@ -480,14 +480,17 @@ class IldlImpl : public IldlImplBase {
         if (form == 1) {
             if (trans)
                 ildl_sub_stream<T, W>(s, D.nrb_ct, D.rb_ct, D.lct_ptr, D.lct_idx, D.lct_val, x,
-                                      y);
+                                      y, xcd_ & 1);
             else
-                ildl_sub_stream<T, W>(s, D.nrb_c, D.rb_c, D.lc_ptr, D.lc_idx, D.lc_val, x, y);
+                ildl_sub_stream<T, W>(s, D.nrb_c, D.rb_c, D.lc_ptr, D.lc_idx, D.lc_val, x, y,
+                                      xcd_ & 1);
         } else if (form > 0) {
             if (trans)
-                ildl_sub_csr<T, W>(s, form, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, x, y);
+                ildl_sub_csr<T, W>(s, form, D.nB, D.lct_ptr, D.lct_idx, D.lct_val, x, y,
+                                   xcd_ & 1);
             else
-                ildl_sub_csr<T, W>(s, form, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, x, y);
+                ildl_sub_csr<T, W>(s, form, D.nC, D.lc_ptr, D.lc_idx, D.lc_val, x, y,
+                                   xcd_ & 1);
         } else {
             const SellDev<T> &M = trans ? D.sct : D.sc;
             ildl_sub_sell<T, W>(s, M.v, M.val, x, y, xcd_);

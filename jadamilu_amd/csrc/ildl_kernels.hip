@@ -11,6 +11,7 @@
 // iteration there, while the vector traffic is small next to the
 // factor's and the extra arithmetic is free in these HBM-bound loops.
 #include "ildl_kernels.hpp"
+#include "xcd.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -20,17 +21,8 @@ namespace jdhip {
 static constexpr int kB = 256;
 static constexpr int kLpr = 4;
 
-// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
-// XCD b % 8), each with its own L2.  With xcd != 0 the kernels below use the
-// block id remapped so that every XCD works on one contiguous range of
-// slices: neighbouring rows share x entries, and the shared cache lines then
-// stay in one L2 instead of being fetched by all eight.
-__device__ __forceinline__ long logical_block(int xcd) {
-    if (!xcd) return blockIdx.x;
-    const unsigned nb = gridDim.x, b = blockIdx.x;
-    const unsigned per = nb / 8, rem = nb % 8, x = b % 8, i = b / 8;
-    return (long)x * per + (x < rem ? x : rem) + i;
-}
+// With xcd != 0 the kernels below take their block id from logical_block()
+// (xcd.hpp): every XCD works on one contiguous range of rows or slices.
 
 static inline int blocks_for(long n, int per_block, int cap = 8192) {
     long b = (n + per_block - 1) / per_block;
@@ -153,9 +145,9 @@ template <typename T, typename W, int LPR>
 __global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
                                                    const T *__restrict__ val, const W *b,
-                                                   const W *xin, W *xout) {
+                                                   const W *xin, W *xout, int xcd) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
+    for (long r = logical_block(xcd) * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
 #pragma unroll 4
         for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR)
@@ -170,9 +162,10 @@ template <typename T, typename W, int LPR>
 __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict__ ptr,
                                                 const int *__restrict__ idx,
                                                 const T *__restrict__ val,
-                                                const W *__restrict__ x, W *__restrict__ y) {
+                                                const W *__restrict__ x, W *__restrict__ y,
+                                                int xcd) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    for (long r = (long)blockIdx.x * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
+    for (long r = logical_block(xcd) * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
         double s = 0.0;
 #pragma unroll 4
         for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[idx[k]];
@@ -193,7 +186,7 @@ __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict
 static constexpr int kStreamCap = 2048;
 
 template <typename T, typename W>
-__device__ __forceinline__ void stream_rows(const int *__restrict__ rblk,
+__device__ __forceinline__ void stream_rows(long blk, const int *__restrict__ rblk,
                                             const int *__restrict__ ptr,
                                             const int *__restrict__ idx,
                                             const T *__restrict__ val,
@@ -202,7 +195,7 @@ __device__ __forceinline__ void stream_rows(const int *__restrict__ rblk,
     __shared__ double prod[kStreamCap];
     __shared__ double red[kB / 64];
     const int tid = threadIdx.x;
-    const int r0 = rblk[blockIdx.x], r1 = rblk[blockIdx.x + 1];
+    const int r0 = rblk[blk], r1 = rblk[blk + 1];
     const int e0 = ptr[r0], e1 = ptr[r1];
     const int ne = e1 - e0;
     owner = false;
@@ -254,11 +247,11 @@ __global__ __launch_bounds__(kB) void k_jacobi_stream(const int *__restrict__ rb
                                                       const int *__restrict__ ptr,
                                                       const int *__restrict__ idx,
                                                       const T *__restrict__ val, const W *b,
-                                                      const W *xin, W *xout) {
+                                                      const W *xin, W *xout, int xcd) {
     int row = 0;
     double s = 0.0;
     bool owner = false;
-    stream_rows(rblk, ptr, idx, val, xin, row, s, owner);
+    stream_rows(logical_block(xcd), rblk, ptr, idx, val, xin, row, s, owner);
     if (owner) xout[row] = (W)((double)b[row] - s);
 }
 
@@ -267,11 +260,12 @@ __global__ __launch_bounds__(kB) void k_sub_stream(const int *__restrict__ rblk,
                                                    const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
                                                    const T *__restrict__ val,
-                                                   const W *__restrict__ x, W *__restrict__ y) {
+                                                   const W *__restrict__ x, W *__restrict__ y,
+                                                   int xcd) {
     int row = 0;
     double s = 0.0;
     bool owner = false;
-    stream_rows(rblk, ptr, idx, val, x, row, s, owner);
+    stream_rows(logical_block(xcd), rblk, ptr, idx, val, x, row, s, owner);
     if (owner) y[row] = (W)((double)y[row] - s);
 }
 
@@ -359,11 +353,11 @@ void ildl_sub_sell(hipStream_t s, const SellView &M, const T *val, const W *x, W
 }
 template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                     const T *val, const W *b, const W *xin, W *xout) {
+                     const T *val, const W *b, const W *xin, W *xout, int xcd) {
     if (nrows <= 0) return;
 #define JDHIP_JC(L)                                                                     \
     k_jacobi_csr<T, W, L><<<blocks_for(nrows, kB / L, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, \
-                                                                            val, b, xin, xout)
+                                                                            val, b, xin, xout, xcd)
     if (lpr >= 64) JDHIP_JC(64);
     else if (lpr >= 32) JDHIP_JC(32);
     else if (lpr >= 16) JDHIP_JC(16);
@@ -373,11 +367,11 @@ void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const in
 }
 template <typename T, typename W>
 void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                  const T *val, const W *x, W *y) {
+                  const T *val, const W *x, W *y, int xcd) {
     if (nrows <= 0) return;
 #define JDHIP_SC(L)                                                                      \
     k_sub_csr<T, W, L><<<blocks_for(nrows, kB / L, 1 << 20), kB, 0, s>>>(nrows, ptr, idx, val, \
-                                                                         x, y)
+                                                                         x, y, xcd)
     if (lpr >= 64) JDHIP_SC(64);
     else if (lpr >= 32) JDHIP_SC(32);
     else if (lpr >= 16) JDHIP_SC(16);
@@ -387,15 +381,16 @@ void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *
 }
 template <typename T, typename W>
 void ildl_jacobi_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr,
-                        const int *idx, const T *val, const W *b, const W *xin, W *xout) {
+                        const int *idx, const T *val, const W *b, const W *xin, W *xout,
+                        int xcd) {
     if (nblk <= 0) return;
-    k_jacobi_stream<T, W><<<nblk, kB, 0, s>>>(rblk, ptr, idx, val, b, xin, xout);
+    k_jacobi_stream<T, W><<<nblk, kB, 0, s>>>(rblk, ptr, idx, val, b, xin, xout, xcd);
 }
 template <typename T, typename W>
 void ildl_sub_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr, const int *idx,
-                     const T *val, const W *x, W *y) {
+                     const T *val, const W *x, W *y, int xcd) {
     if (nblk <= 0) return;
-    k_sub_stream<T, W><<<nblk, kB, 0, s>>>(rblk, ptr, idx, val, x, y);
+    k_sub_stream<T, W><<<nblk, kB, 0, s>>>(rblk, ptr, idx, val, x, y, xcd);
 }
 std::vector<int> stream_row_blocks(int nrows, const int *ptr) {
     std::vector<int> rb;
@@ -464,15 +459,15 @@ void ildl_dense_gemv(hipStream_t s, int m, const T *Minv, const W *x, W *y) {
     template void ildl_sub_sell<T, W>(hipStream_t, const SellView &, const T *, const W *,    \
                                       W *, int);                                             \
     template void ildl_jacobi_csr<T, W>(hipStream_t, int, int, const int *, const int *,      \
-                                        const T *, const W *, const W *, W *);               \
+                                        const T *, const W *, const W *, W *, int);          \
     template void ildl_sub_csr<T, W>(hipStream_t, int, int, const int *, const int *,         \
-                                     const T *, const W *, W *);                             \
+                                     const T *, const W *, W *, int);                        \
     template void ildl_dense_gemv<T, W>(hipStream_t, int, const T *, const W *, W *);         \
     template void ildl_jacobi_stream<T, W>(hipStream_t, int, const int *, const int *,        \
                                            const int *, const T *, const W *, const W *,      \
-                                           W *);                                              \
+                                           W *, int);                                         \
     template void ildl_sub_stream<T, W>(hipStream_t, int, const int *, const int *,           \
-                                        const int *, const T *, const W *, W *);
+                                        const int *, const T *, const W *, W *, int);
 INST(double, double)
 INST(float, double)
 INST(float, float)

@@ -52,26 +52,29 @@ struct SellView {
 template <typename T, typename W>
 void ildl_jacobi_sweep_sell(hipStream_t s, const SellView &M, const T *val, const W *b,
                             const W *xin, W *xout, int xcd);
-// CSR forms with lpr (4, 8, 16, 32 or 64) lanes per row, for matrices with long rows
+// CSR forms with lpr (4, 8, 16, 32 or 64) lanes per row, for matrices with long rows;
+// xcd != 0: XCD-contiguous block order (as for the SELL and stream forms)
 template <typename T, typename W>
 void ildl_jacobi_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                     const T *val, const W *b, const W *xin, W *xout);
+                     const T *val, const W *b, const W *xin, W *xout, int xcd);
 template <typename T, typename W>
 void ildl_sub_csr(hipStream_t s, int lpr, int nrows, const int *ptr, const int *idx,
-                  const T *val, const W *x, W *y);
+                  const T *val, const W *x, W *y, int xcd);
 // y[r] -= sum_j M(r, j) x[j] over the rows of a SELL-C-sigma matrix
 template <typename T, typename W>
 void ildl_sub_sell(hipStream_t s, const SellView &M, const T *val, const W *x, W *y, int xcd);
 // CSR-stream form: blocks of consecutive rows with at most 2048 entries
 // (and 256 rows) each, or one longer row; rblk[b] = first row of block b,
 // rblk[nblk] = nrows.  Entries are streamed through LDS with coalesced loads.
+// xcd != 0: every XCD takes one contiguous range of row blocks.
 std::vector<int> stream_row_blocks(int nrows, const int *ptr);
 template <typename T, typename W>
 void ildl_jacobi_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr,
-                        const int *idx, const T *val, const W *b, const W *xin, W *xout);
+                        const int *idx, const T *val, const W *b, const W *xin, W *xout,
+                        int xcd);
 template <typename T, typename W>
 void ildl_sub_stream(hipStream_t s, int nblk, const int *rblk, const int *ptr, const int *idx,
-                     const T *val, const W *x, W *y);
+                     const T *val, const W *x, W *y, int xcd);
 template <typename W>
 void ildl_dscale(hipStream_t s, int n, const double *dinv, W *x);
 // x = D^-1 x with 2x2 blocks: row i pairs with pair[i] (-1: 1x1 pivot,

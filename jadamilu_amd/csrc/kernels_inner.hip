@@ -7,6 +7,7 @@
 // that turns the rest of the step into no-ops (the reference's early exits).
 #include "kernels.hpp"
 #include "reduce.hpp"
+#include "xcd.hpp"
 
 #include <algorithm>
 
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
     constexpr int GPB = kBlock / LPR;
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     double acc[1] = {0.0};
-    for (long row = (long)blockIdx.x * GPB + g; row < n; row += (long)gridDim.x * GPB) {
+    for (long row = logical_block(1) * GPB + g; row < n; row += (long)gridDim.x * GPB) {
         const int b = rowptr[row], e = rowptr[row + 1];
         double s = 0.0;
         for (int k = b + l; k < e; k += LPR) s += (double)val[k] * (double)x[col[k]];
@@ -150,7 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     constexpr int WPB = kBlock / 64;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double acc[1] = {0.0};
-    for (long sl = (long)blockIdx.x * WPB + wv; sl < nslices; sl += (long)gridDim.x * WPB) {
+    // XCD-contiguous slice ranges: a slice's x gathers overlap its
+    // neighbours', so they are served from one L2
+    for (long sl = logical_block(1) * WPB + wv; sl < nslices; sl += (long)gridDim.x * WPB) {
         const long row = sl * 64 + lane;
         if (row >= n) continue;
         const int code = wm[sl];

@@ -12,6 +12,7 @@
 // matrix for k separate SpMVs.  Vectors are column-major, X(i, j) =
 // x[i + j*ldx]; blocks wider than 8 run in passes of 8.
 #include "kernels.hpp"
+#include "xcd.hpp"
 
 namespace jdhip {
 
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sell(int n, int nslices,
                                                       double *__restrict__ y, long ldy) {
     constexpr int WPB = kBlock / 64;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (long sl = (long)blockIdx.x * WPB + wv; sl < nslices; sl += (long)gridDim.x * WPB) {
+    for (long sl = logical_block(1) * WPB + wv; sl < nslices; sl += (long)gridDim.x * WPB) {
         const long row = sl * 64 + lane;
         if (row >= n) continue;
         const int code = wm[sl];
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_csr(int n, const int *__restric
                                                      double *__restrict__ y, long ldy) {
     constexpr int LPR = 8, GPB = kBlock / LPR;
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    for (long row = (long)blockIdx.x * GPB + g; row < n; row += (long)gridDim.x * GPB) {
+    for (long row = logical_block(1) * GPB + g; row < n; row += (long)gridDim.x * GPB) {
         double s[KB];
 #pragma unroll
         for (int j = 0; j < KB; ++j) s[j] = 0.0;
