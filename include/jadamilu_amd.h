@@ -11,6 +11,14 @@
  *   dpjdrevcom_  DPJDREVCOM  src/jd/PJD.F:268-362    (jadamilu1.h:21-27)
  *   djdrevcom_   DJDREVCOM   src/jd/PJD.F:369-427    (jadamilu1.h:29-35)
  *   dpjdcleanup_ DPJDCLEANUP src/jd/PJD.F:435-485    (jadamilu1.h:28)
+ *   dpjd_gep_       DPJD_GEP       src/jd/PJD.F:492-570   (jadamilu1.h:36-43)
+ *   dpjdrevcom_gep_ DPJDREVCOM_GEP src/jd/PJD.F:576-703   (jadamilu1.h:44-53)
+ *   djdrevcom_gep_  DJDREVCOM_GEP  src/jd/PJD.F:709-798   (jadamilu1.h:54-59)
+ * The generalized drivers solve A x = lambda B x with B symmetric positive
+ * definite, for ISEARCH <= 1 (ISEARCH = 2 returns INFO = -8); their reverse
+ * communication adds NDX3 and the codes IJOB = 1 (X(NDX2) = A X(NDX1) and
+ * X(NDX3) = B X(NDX1)), 3 (A only) and 4 (X(NDX3) = B X(NDX1) only); a mass
+ * matrix found not positive definite ends the run with INFO = -54322.
  * The un-underscored names (dpjd, dpjdrevcom, djdrevcom, dpjdcleanup) are
  * exported as aliases, matching a reference build without -D__UNDERSCORE__
  * (src/include/namesjadamilu.h:53-88).
@@ -65,6 +73,33 @@ void djdrevcom(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
                int *ndx1, int *ndx2, int *iprint, int *info, double *gap);
 void dpjdcleanup(void);
 
+void dpjd_gep_(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib, double *eigs,
+               double *res, double *x, int *lx, int *neig, double *sigma, int *isearch,
+               int *ninit, int *madspace, int *iter, double *tol, double *shift,
+               double *droptol, double *mem, int *icntl, int *iprint, int *info, double *gap);
+void dpjdrevcom_gep_(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib,
+                     double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+                     int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                     double *shift, double *droptol, double *mem, int *icntl, int *ijob,
+                     int *ndx1, int *ndx2, int *ndx3, int *iprint, int *info, double *gap);
+void djdrevcom_gep_(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                    double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+                    double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
+                    int *info, double *gap);
+void dpjd_gep(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib, double *eigs,
+              double *res, double *x, int *lx, int *neig, double *sigma, int *isearch,
+              int *ninit, int *madspace, int *iter, double *tol, double *shift,
+              double *droptol, double *mem, int *icntl, int *iprint, int *info, double *gap);
+void dpjdrevcom_gep(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib,
+                    double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+                    int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                    double *shift, double *droptol, double *mem, int *icntl, int *ijob,
+                    int *ndx1, int *ndx2, int *ndx3, int *iprint, int *info, double *gap);
+void djdrevcom_gep(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                   double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+                   double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
+                   int *info, double *gap);
+
 /* ---- context API: device-resident X ------------------------------------- */
 typedef struct jdhip_ctx jdhip_ctx;
 
@@ -90,6 +125,27 @@ void jdhip_jdrevcom(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_
                     int *neig, double sigma, int isearch, int ninit, int madspace, int *iter,
                     double tol, int *ijob, long *ndx1, long *ndx2, int iprint, int *info,
                     double *gap);
+/* As DPJD_GEP (PJD.F:492): A x = lambda B x, both matrices 1-based upper
+ * CSR with every diagonal entry, x a DEVICE array.  The preconditioner
+ * factors A - SHIFT B (symilupackfactorgep.c), the default shift is the
+ * estimate of SYMILUPACKSAVEDIAGGEP (symamgsavediaggep.c). */
+void jdhip_pjd_gep(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *b, int *jb,
+                   int *ib, double *eigs, double *res, double *x_dev, long lx, int *neig,
+                   double sigma, int isearch, int ninit, int madspace, int *iter, double tol,
+                   double *shift, double *droptol, double *mem, int *icntl, int iprint,
+                   int *info, double *gap);
+/* As DPJDREVCOM_GEP (PJD.F:576); NDX (1-based) index the DEVICE array. */
+void jdhip_pjdrevcom_gep(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *b,
+                         int *jb, int *ib, double *eigs, double *res, double *x_dev, long lx,
+                         int *neig, double sigma, int isearch, int ninit, int madspace,
+                         int *iter, double tol, double *shift, double *droptol, double *mem,
+                         int *icntl, int *ijob, long *ndx1, long *ndx2, long *ndx3,
+                         int iprint, int *info, double *gap);
+/* As DJDREVCOM_GEP (PJD.F:709); NDX (1-based) index the DEVICE array. */
+void jdhip_jdrevcom_gep(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_dev,
+                        long lx, int *neig, double sigma, int isearch, int ninit, int madspace,
+                        int *iter, double tol, int *ijob, long *ndx1, long *ndx2, long *ndx3,
+                        int iprint, int *info, double *gap);
 /* As DPJDCLEANUP (PJD.F:435): release the preconditioner of ctx. */
 void jdhip_cleanup(jdhip_ctx *ctx);
 

@@ -59,6 +59,13 @@ SIGNATURES = {
     "dpjdcleanup_": (None, []),
     "dpjd": (None, None), "dpjdrevcom": (None, None), "djdrevcom": (None, None),
     "dpjdcleanup": (None, []),
+    "dpjd_gep_": (None, [_i, _d, _i, _i, _d, _i, _i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i,
+                         _d, _d, _d, _d, _i, _i, _i, _d]),
+    "dpjdrevcom_gep_": (None, [_i, _d, _i, _i, _d, _i, _i, _d, _d, _d, _i, _i, _d, _i, _i, _i,
+                               _i, _d, _d, _d, _d, _i, _i, _i, _i, _i, _i, _i, _d]),
+    "djdrevcom_gep_": (None, [_i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _d, _i, _i, _i, _i,
+                              _i, _i, _d]),
+    "dpjd_gep": (None, None), "dpjdrevcom_gep": (None, None), "djdrevcom_gep": (None, None),
     "jdhip_create": (_v, [C.c_int]),
     "jdhip_destroy": (None, [_v]),
     "jdhip_pjd": (None, [_v, C.c_int, _d, _i, _i, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
@@ -68,6 +75,15 @@ SIGNATURES = {
                                _l, _l, C.c_int, _i, _d]),
     "jdhip_jdrevcom": (None, [_v, C.c_int, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
                               C.c_int, C.c_int, _i, C.c_double, _i, _l, _l, C.c_int, _i, _d]),
+    "jdhip_pjd_gep": (None, [_v, C.c_int, _d, _i, _i, _d, _i, _i, _d, _d, _v, C.c_long, _i,
+                             C.c_double, C.c_int, C.c_int, C.c_int, _i, C.c_double, _d, _d, _d,
+                             _i, C.c_int, _i, _d]),
+    "jdhip_pjdrevcom_gep": (None, [_v, C.c_int, _d, _i, _i, _d, _i, _i, _d, _d, _v, C.c_long,
+                                   _i, C.c_double, C.c_int, C.c_int, C.c_int, _i, C.c_double,
+                                   _d, _d, _d, _i, _i, _l, _l, _l, C.c_int, _i, _d]),
+    "jdhip_jdrevcom_gep": (None, [_v, C.c_int, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
+                                  C.c_int, C.c_int, _i, C.c_double, _i, _l, _l, _l, C.c_int,
+                                  _i, _d]),
     "jdhip_cleanup": (None, [_v]),
     "jdhip_reset": (C.c_int, [_v]),
     "jdhip_order_amd": (C.c_int, [C.c_int, _i, _i, _i]),

@@ -48,6 +48,7 @@ struct SolverOps : Ops {
     bool has_f32() const override { return f32; }
     void matvec_f(const float *x, float *y, double eta, const float *u, double *udot) override;
     void precond_f(JdEngine &e, float *z, const float *r) override;
+    void bmatvec(const double *x, double *y) override;
 };
 
 struct Solver {
@@ -103,17 +104,25 @@ struct Solver {
     // device operator for internal matvecs
     CsrDev A;
     int A_n = 0;
+    // mass matrix B of the generalized problem (internal B matvecs)
+    CsrDev Bm;
+    int B_n = 0;
+    double B_fp = 0.0;
     std::vector<int> A_host_ia;  // identity check for re-upload
     // host-X mirror of the legacy API
     double *dX = nullptr;
     long dX_len = 0;
-    int pend_kind = 0;  // 1 matvec, 2 precond
-    long pend1 = 0, pend2 = 0;
+    int pend_kind = 0;  // 1 matvec, 2 precond, 3 A only, 4 B only (generalized)
+    long pend1 = 0, pend2 = 0, pend3 = 0;
     // pointers of the current call (PJDRVCOM arguments)
     int n = 0, isearch = 0;
     double sigma = 0;
     double *a = nullptr, *eigs = nullptr, *shift = nullptr, *droptol = nullptr, *mem = nullptr;
     int *ja = nullptr, *ia = nullptr, *neig = nullptr, *icntl = nullptr;
+    // generalized problem A x = lambda B x (B upper CSR, 1-based)
+    bool gep = false;
+    double *b = nullptr;
+    int *jb = nullptr, *ib = nullptr;
     std::vector<double> shifts_log, droptols_log, fills_log;
     int nsetups = 0;
     double precond_seconds = 0.0;
@@ -125,18 +134,22 @@ struct Solver {
     ~Solver() {
         release_precond();
         free_matrix();
+        free_csr(Bm);
         if (dX) hipFree(dX);
         ctx.release();
     }
-    void free_matrix() {
-        if (A.rowptr) hipFree(A.rowptr);
-        if (A.col) hipFree(A.col);
-        if (A.val) hipFree(A.val);
-        if (A.valf) hipFree(A.valf);
-        for (void *p : {(void *)A.s_vptr, (void *)A.s_iptr, (void *)A.s_wm, (void *)A.s_val,
-                        (void *)A.s_valf, (void *)A.s_idx})
+    static void free_csr(CsrDev &M) {
+        if (M.rowptr) hipFree(M.rowptr);
+        if (M.col) hipFree(M.col);
+        if (M.val) hipFree(M.val);
+        if (M.valf) hipFree(M.valf);
+        for (void *p : {(void *)M.s_vptr, (void *)M.s_iptr, (void *)M.s_wm, (void *)M.s_val,
+                        (void *)M.s_valf, (void *)M.s_idx})
             if (p) hipFree(p);
-        A = CsrDev{};
+        M = CsrDev{};
+    }
+    void free_matrix() {
+        free_csr(A);
         A_n = 0;
     }
     void release_precond() {
@@ -154,8 +167,19 @@ struct Solver {
         dX_len = lx;
     }
 
-    void upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr);
-    void build_sell(int n_, const std::vector<int> &rp, const std::vector<int> &col,
+    void upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr) {
+        free_matrix();
+        upload_csr(A, n_, ia_, ja_, a_, lpr);
+        A_n = n_;
+    }
+    void upload_b(int n_, const int *ib_, const int *jb_, const double *b_) {
+        free_csr(Bm);
+        upload_csr(Bm, n_, ib_, jb_, b_, 0);
+        B_n = n_;
+    }
+    void upload_csr(CsrDev &M, int n_, const int *ia_, const int *ja_, const double *a_,
+                    int lpr);
+    void build_sell(CsrDev &M, int n_, const std::vector<int> &rp, const std::vector<int> &col,
                     const std::vector<double> &val);
     double A_fp = 0.0;
     static double fingerprint(int n_, const int *ia_, const int *ja_, const double *a_) {
@@ -165,12 +189,15 @@ struct Solver {
                 s += (double)ja_[k] * 1e-7 + a_[k] * (1.0 + 1e-9 * (double)(k % 977));
         return s;
     }
-    int autotune_lpr();
+    int autotune_lpr(CsrDev &M);
 
     // PJDINIT (JDaux.F:549-1325)
     void pjdinit(int isearch_, double sigma_, double shift_in);
     void pjdinit_ilu(int isearch_, double sigma_);
     double savediag_shift() const;
+    double savediag_shift_gep() const;
+    void shifted_pencil(double sh, std::vector<int> &mia, std::vector<int> &mja,
+                        std::vector<double> &ma) const;
     void suggest(int NE, const double *ev, double gp);
     double tol_ = 1e-10;
     double *gap_ = nullptr;
@@ -193,7 +220,7 @@ struct Solver {
                   double *x_dev, long lx, int *neig_, double sigma_, int isearch_, int ninit,
                   int madspace, int *iter, double tol, double *shift_, double *droptol_,
                   double *mem_, int *icntl_, int *ijob, long *ndx1, long *ndx2, int iprint,
-                  int *info, double *gap, bool internal_mv);
+                  int *info, double *gap, bool internal_mv, long *ndx3 = nullptr);
 };
 
 void SolverOps::matvec(const double *x, double *y, const double *eta_dev, double eta,
@@ -207,12 +234,15 @@ void SolverOps::matvec_f(const float *x, float *y, double eta, const float *u, d
     spmv<float>(S->ctx.stream, S->ctx.rb, S->A, x, y, eta, u, udot);
 }
 void SolverOps::precond_f(JdEngine &, float *z, const float *r) { S->precond_request_f(z, r); }
+void SolverOps::bmatvec(const double *x, double *y) {
+    spmv<double>(S->ctx.stream, S->ctx.rb, S->Bm, x, y, 0.0, nullptr, nullptr);
+}
 
 // Full (both triangles) CSR of the symmetric matrix, built on the host from
 // the upper triangle and uploaded once: the SpMV then reads each row with
 // coalesced loads and needs no atomics.
-void Solver::upload_matrix(int n_, const int *ia_, const int *ja_, const double *a_, int lpr) {
-    free_matrix();
+void Solver::upload_csr(CsrDev &A, int n_, const int *ia_, const int *ja_, const double *a_,
+                        int lpr) {
     std::vector<int> cnt(n_ + 1, 0);
     for (int i = 0; i < n_; ++i)
         for (int k = ia_[i] - 1; k < ia_[i + 1] - 1; ++k) {
@@ -254,19 +284,18 @@ void Solver::upload_matrix(int n_, const int *ia_, const int *ja_, const double 
     }
     A.n = n_;
     A.nnz = nnz;
-    A_n = n_;
-    build_sell(n_, cnt, col, val);
+    build_sell(A, n_, cnt, col, val);
     A.lpr = lpr > 0 ? lpr : 8;
     if (lpr < 0 && A.nslices > 0) A.lpr = 0;
-    if (lpr == 0) A.lpr = autotune_lpr();
+    if (lpr == 0) A.lpr = autotune_lpr(A);
 }
 
 // Sliced-ELL copy of the full CSR: slices of 64 rows, entries of slice s at
 // element offsets vptr[s] + k*64 + lane (k < width of the slice); padding
 // holds value 0 at the row's own column.  Skipped when padding would more
 // than double the stored entries.
-void Solver::build_sell(int n_, const std::vector<int> &rp, const std::vector<int> &col,
-                        const std::vector<double> &val) {
+void Solver::build_sell(CsrDev &A, int n_, const std::vector<int> &rp,
+                        const std::vector<int> &col, const std::vector<double> &val) {
     const int ns = (n_ + 63) / 64;
     std::vector<int> wm(ns);
     std::vector<long> vptr(ns + 1, 0), iptr(ns + 1, 0);
@@ -326,7 +355,7 @@ void Solver::build_sell(int n_, const std::vector<int> &rp, const std::vector<in
 }
 
 // Pick lanes-per-row by timing the candidates on this matrix.
-int Solver::autotune_lpr() {
+int Solver::autotune_lpr(CsrDev &A) {
     if (A.n < 4096) return 4;
     double *x = nullptr, *y = nullptr;
     JDHIP_CHECK(hipMalloc(&x, sizeof(double) * A.n));
@@ -456,6 +485,87 @@ double Solver::savediag_shift() const {
     return val;
 }
 
+// SYMILUPACKSAVEDIAGGEP (src/ilupack/symamgsavediaggep.c:13-139): the same
+// Gershgorin-type lower estimate for A x = lambda B x, on D A D with D the
+// inverse square root of a diagonal that dominates B (each off-diagonal b_ij
+// lumped onto b_ii and b_jj with the weights sqrt(b_ii/b_jj) and its
+// inverse, in row order as the values change).
+double Solver::savediag_shift_gep() const {
+    std::vector<double> dg(n, 0.0), disc(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = ib[i] - 1; k < ib[i + 1] - 1; ++k)
+            if (jb[k] - 1 == i) dg[i] = b[k];
+    for (int i = 0; i < n; ++i)
+        for (int k = ib[i] - 1; k < ib[i + 1] - 1; ++k) {
+            const int j = jb[k] - 1;
+            if (j == i) continue;
+            const double v = std::sqrt(dg[i] / dg[j]);
+            dg[i] += std::fabs(b[k]) * v;
+            dg[j] += std::fabs(b[k]) / v;
+        }
+    for (int i = 0; i < n; ++i) dg[i] = 1.0 / std::sqrt(dg[i]);
+    double ti = 0.0;
+    int ki = 0;
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+            const int j = ja[k] - 1;
+            if (j == i) {
+                const double v = dg[i] * a[k] * dg[j];
+                if (i == 0) {
+                    ti = v;
+                } else if (v < ti) {
+                    ti = v;
+                    ki = i;
+                }
+            } else {
+                const double v = dg[i] * std::fabs(a[k]) * dg[j];
+                disc[i] += v;
+                disc[j] += v;
+            }
+        }
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+            if (ja[k] - 1 == i) dg[i] = dg[i] * a[k] * dg[i];
+    double val = ti - disc[ki];
+    for (int i = 0; i < n; ++i) {
+        const double t = dg[i] - disc[i];
+        if (t < val && dg[i] - ti < disc[ki]) val = t;
+    }
+    return val;
+}
+
+// A - sh B as a 1-based upper CSR with ascending columns and every
+// diagonal entry (the union of the two patterns).
+void Solver::shifted_pencil(double sh, std::vector<int> &mia, std::vector<int> &mja,
+                            std::vector<double> &ma) const {
+    mia.assign(n + 1, 1);
+    mja.clear();
+    ma.clear();
+    std::vector<double> acc(n, 0.0);
+    std::vector<int> mark(n, -1), cols;
+    for (int i = 0; i < n; ++i) {
+        cols.clear();
+        auto add = [&](int j, double v) {
+            if (mark[j] != i) {
+                mark[j] = i;
+                acc[j] = v;
+                cols.push_back(j);
+            } else {
+                acc[j] += v;
+            }
+        };
+        add(i, 0.0);
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) add(ja[k] - 1, a[k]);
+        for (int k = ib[i] - 1; k < ib[i + 1] - 1; ++k) add(jb[k] - 1, -sh * b[k]);
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) {
+            mja.push_back(j + 1);
+            ma.push_back(acc[j]);
+        }
+        mia[i + 1] = (int)mja.size() + 1;
+    }
+}
+
 // PJDINIT, ILU branch (JDaux.F:719-1323): shift choice, factorisation of
 // A - shift0 I, retries with a modified DROPTOL on failure, conversion to an
 // SPD preconditioner (ISEARCH <= 1) and restart with a smaller shift when too
@@ -475,7 +585,7 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
             for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
                 if (ja[k] == i + 1) dmin = (i == 0) ? a[k] : std::min(dmin, a[k]);
         diagmin = dmin;
-        shift0 = savediag_shift();
+        shift0 = gep ? savediag_shift_gep() : savediag_shift();
         if (isearch_ >= 2) shift0 = sigma_;
         int ggsh = 0;
         if (isearch_ == 1 && *shift < diagmin) shift0 = *shift;
@@ -494,7 +604,16 @@ void Solver::pjdinit_ilu(int isearch_, double sigma_) {
         opt.dense_max = ildl_dense_max;
         opt.threads = ildl_threads;
         IldlHost h;
-        ildl_factor_host(n, ia, ja, a, shift0, opt, h);
+        if (gep) {
+            // the generalized problem factors A - shift0 B (the merge of the
+            // two upper triangles, symilupackfactorgep.c:133-195)
+            std::vector<int> mia, mja;
+            std::vector<double> ma;
+            shifted_pencil(shift0, mia, mja, ma);
+            ildl_factor_host(n, mia.data(), mja.data(), ma.data(), 0.0, opt, h);
+        } else {
+            ildl_factor_host(n, ia, ja, a, shift0, opt, h);
+        }
         if (fail_setup > 0 && nsetups + 1 == fail_setup) h.res.ierr = -1;
         const IldlResult &r = h.res;
         memused = r.fill;
@@ -689,7 +808,8 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
                       double *x_dev, long lx, int *neig_, double sigma_, int isearch_,
                       int ninit, int madspace, int *iter, double tol, double *shift_,
                       double *droptol_, double *mem_, int *icntl_, int *ijob, long *ndx1,
-                      long *ndx2, int iprint, int *info, double *gap, bool internal_mv) {
+                      long *ndx2, int iprint, int *info, double *gap, bool internal_mv,
+                      long *ndx3) {
     n = n_;
     a = a_;
     ja = ja_;
@@ -734,7 +854,7 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
                     for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
                         if (ja[k] == i + 1) dmin = (i == 0) ? a[k] : std::min(dmin, a[k]);
                 diagmin = dmin;
-                shift0 = savediag_shift();
+                shift0 = gep ? savediag_shift_gep() : savediag_shift();
                 if (isearch >= 2) shift0 = sigma;
                 if (isearch == 1 && *shift < diagmin) shift0 = *shift;
                 eng.timeilu = 0.0;
@@ -756,8 +876,9 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
         eng.set_ops(&ops);
     }
     try {
+        long ndx3_dummy = 0;
         eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol,
-                 ijob, ndx1, ndx2, iprint, info, gap);
+                 ijob, ndx1, ndx2, ndx3 ? ndx3 : &ndx3_dummy, iprint, info, gap, gep);
     } catch (const PrecondSetupFailed &) {
         // JDaux.F:431-441 ends the run here.  The reference marks the
         // diagonal preconditioner as current (factdgl = 1) without building
@@ -769,7 +890,9 @@ void Solver::pjdrvcom(int n_, double *a_, int *ja_, int *ia_, double *eigs_, dou
         *ijob = 0;
         return;
     }
-    if (*ijob == 3) *ijob = 1;
+    // the standard drivers report the final-check products as IJOB = 1; the
+    // generalized ones keep IJOB = 3 (A only) and 4 (B only) (PJD.F:360, 427)
+    if (*ijob == 3 && !gep) *ijob = 1;
     if (*ijob == 0 && ierr == 0) suggest(*neig, eigs, *gap);
 }
 
@@ -897,6 +1020,7 @@ void jdhip_pjdrevcom(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double 
                      long *ndx2, int iprint, int *info, double *gap) {
     try {
         long l1 = *ndx1 - 1, l2 = *ndx2 - 1;
+        if (*ijob == 0) ctx->S->gep = false;
         ctx->S->pjdrvcom(n, a, ja, ia, eigs, res, x_dev, lx, neig, sigma, isearch, ninit,
                          madspace, iter, tol, shift, droptol, mem, icntl, ijob, &l1, &l2,
                          iprint, info, gap, false);
@@ -917,8 +1041,9 @@ void jdhip_jdrevcom(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_
         Solver &S = *ctx->S;
         long l1 = *ndx1 - 1, l2 = *ndx2 - 1;
         if (*ijob == 0) S.eng.set_ops(nullptr);
+        long l3 = 0;
         S.eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol,
-                   ijob, &l1, &l2, iprint, info, gap);
+                   ijob, &l1, &l2, &l3, iprint, info, gap);
         if (*ijob == 3) *ijob = 1;
         *ndx1 = l1 + 1;
         *ndx2 = l2 + 1;
@@ -955,6 +1080,7 @@ void jdhip_pjd(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *eigs,
         icntl[0] = 1;
         int ijob = 0;
         long l1 = 0, l2 = 0;
+        S.gep = false;
         S.pjdrvcom(n, a, ja, ia, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace,
                    iter, tol, shift, droptol, mem, icntl, &ijob, &l1, &l2, iprint, info, gap,
                    true);
@@ -962,6 +1088,112 @@ void jdhip_pjd(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *eigs,
     } catch (const std::exception &e) {
         set_err(ctx, e.what());
         *info = JDHIP_INFO_DEVICE_ERROR;
+    }
+}
+
+// ---- generalized problem A x = lambda B x (PJD.F:492-703) ----
+
+// JDaux.F:215-229 for A and B: the diagonal entry moves to the front of each row
+static void diag_first(int n, double *a, int *ja, const int *ia) {
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+            if (ja[k] == i + 1) {
+                int l = ia[i] - 1;
+                std::swap(ja[k], ja[l]);
+                std::swap(a[k], a[l]);
+            }
+}
+
+void jdhip_pjd_gep(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *b, int *jb,
+                   int *ib, double *eigs, double *res, double *x_dev, long lx, int *neig,
+                   double sigma, int isearch, int ninit, int madspace, int *iter, double tol,
+                   double *shift, double *droptol, double *mem, int *icntl, int iprint,
+                   int *info, double *gap) {
+    try {
+        Solver &S = *ctx->S;
+        if (ja[0] < 0 || jb[0] < 0) {  // internal products need both matrices
+            *info = -3;
+            return;
+        }
+        diag_first(n, a, ja, ia);
+        diag_first(n, b, jb, ib);
+        double fp = S.fingerprint(n, ia, ja, a);
+        if (!S.A.rowptr || S.A_n != n || S.A_fp != fp) {
+            S.upload_matrix(n, ia, ja, a, 0);
+            S.A_fp = fp;
+        }
+        double fpb = S.fingerprint(n, ib, jb, b);
+        if (!S.Bm.rowptr || S.B_n != n || S.B_fp != fpb) {
+            S.upload_b(n, ib, jb, b);
+            S.B_fp = fpb;
+        }
+        int keep = icntl[0];
+        icntl[0] = 1;
+        int ijob = 0;
+        long l1 = 0, l2 = 0, l3 = 0;
+        S.gep = true;
+        S.b = b;
+        S.jb = jb;
+        S.ib = ib;
+        S.pjdrvcom(n, a, ja, ia, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace,
+                   iter, tol, shift, droptol, mem, icntl, &ijob, &l1, &l2, iprint, info, gap,
+                   true, &l3);
+        icntl[0] = keep;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+    }
+}
+
+void jdhip_pjdrevcom_gep(jdhip_ctx *ctx, int n, double *a, int *ja, int *ia, double *b,
+                         int *jb, int *ib, double *eigs, double *res, double *x_dev, long lx,
+                         int *neig, double sigma, int isearch, int ninit, int madspace,
+                         int *iter, double tol, double *shift, double *droptol, double *mem,
+                         int *icntl, int *ijob, long *ndx1, long *ndx2, long *ndx3,
+                         int iprint, int *info, double *gap) {
+    try {
+        Solver &S = *ctx->S;
+        long l1 = *ndx1 - 1, l2 = *ndx2 - 1, l3 = *ndx3 - 1;
+        if (*ijob == 0) {
+            if (ja[0] >= 0 && jb[0] >= 0) {
+                diag_first(n, a, ja, ia);
+                diag_first(n, b, jb, ib);
+            }
+            S.gep = true;
+            S.b = b;
+            S.jb = jb;
+            S.ib = ib;
+        }
+        S.pjdrvcom(n, a, ja, ia, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace,
+                   iter, tol, shift, droptol, mem, icntl, ijob, &l1, &l2, iprint, info, gap,
+                   false, &l3);
+        *ndx1 = l1 + 1;
+        *ndx2 = l2 + 1;
+        *ndx3 = l3 + 1;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+    }
+}
+
+void jdhip_jdrevcom_gep(jdhip_ctx *ctx, int n, double *eigs, double *res, double *x_dev,
+                        long lx, int *neig, double sigma, int isearch, int ninit, int madspace,
+                        int *iter, double tol, int *ijob, long *ndx1, long *ndx2, long *ndx3,
+                        int iprint, int *info, double *gap) {
+    try {
+        Solver &S = *ctx->S;
+        long l1 = *ndx1 - 1, l2 = *ndx2 - 1, l3 = *ndx3 - 1;
+        if (*ijob == 0) S.eng.set_ops(nullptr);
+        S.eng.step(n, eigs, res, x_dev, lx, neig, sigma, isearch, ninit, madspace, iter, tol,
+                   ijob, &l1, &l2, &l3, iprint, info, gap, true);
+        *ndx1 = l1 + 1;
+        *ndx2 = l2 + 1;
+        *ndx3 = l3 + 1;
+    } catch (const std::exception &e) {
+        set_err(ctx, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
     }
 }
 
@@ -1349,6 +1581,58 @@ static void legacy_revcom(int n, double *x, int lx, int ninit, int nwant, int *i
     }
 }
 
+// host-X wrapper of the generalized revcom drivers: IJOB = 1 (the caller
+// writes A x into X(NDX2) and B x into X(NDX3)), 3 (A x only), 4 (B x only),
+// 2 (preconditioner solve into X(NDX1))
+template <typename Fn>
+static void legacy_revcom_gep(int n, double *x, int lx, int ninit, int nwant, int *ijob,
+                              int *ndx1, int *ndx2, int *ndx3, int *info, int *neig,
+                              Fn &&call) {
+    jdhip_ctx *c = legacy_ctx();
+    if (!c) {
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+        return;
+    }
+    Solver &S = *c->S;
+    try {
+        if (*ijob == 0) {
+            S.ensure_dx(lx);
+            if (ninit > 0) copy_h2d(S, x, 0, std::min<long>((long)ninit * n, lx));
+            S.pend_kind = 0;
+        } else if (S.pend_kind == 1) {
+            copy_h2d(S, x, S.pend2, n);
+            copy_h2d(S, x, S.pend3, n);
+        } else if (S.pend_kind == 3) {
+            copy_h2d(S, x, S.pend2, n);
+        } else if (S.pend_kind == 4) {
+            copy_h2d(S, x, S.pend3, n);
+        } else if (S.pend_kind == 2) {
+            copy_h2d(S, x, S.pend1, n);
+        }
+        long l1 = 0, l2 = 0, l3 = 0;
+        call(c, S.dX, &l1, &l2, &l3);
+        if (*ijob >= 1 && *ijob <= 4) {
+            S.pend_kind = *ijob;
+            S.pend1 = l1 - 1;
+            S.pend2 = l2 - 1;
+            S.pend3 = l3 - 1;
+            copy_d2h(S, x, (*ijob == 2) ? S.pend2 : S.pend1, n);
+            *ndx1 = (int)l1;
+            *ndx2 = (int)l2;
+            *ndx3 = (int)l3;
+        } else {
+            S.pend_kind = 0;
+            long cnt = (long)n * std::max(1, std::min(nwant, *neig + 1));
+            if (*info >= 0 || *neig > 0) copy_d2h(S, x, 0, std::min<long>(cnt, lx));
+        }
+    } catch (const std::exception &e) {
+        set_err(c, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        *ijob = 0;
+    }
+}
+
 extern "C" {
 
 void dpjdrevcom_(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x,
@@ -1405,6 +1689,91 @@ void dpjd_(int *n, double *a, int *ja, int *ia, double *eigs, double *res, doubl
         set_err(c, e.what());
         *info = JDHIP_INFO_DEVICE_ERROR;
     }
+}
+
+void dpjd_gep_(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib, double *eigs,
+               double *res, double *x, int *lx, int *neig, double *sigma, int *isearch,
+               int *ninit, int *madspace, int *iter, double *tol, double *shift,
+               double *droptol, double *mem, int *icntl, int *iprint, int *info, double *gap) {
+    jdhip_ctx *c = legacy_ctx();
+    if (!c) {
+        *info = JDHIP_INFO_DEVICE_ERROR;
+        return;
+    }
+    Solver &S = *c->S;
+    int nwant = *neig;
+    try {
+        S.ensure_dx(*lx);
+        if (*ninit > 0) copy_h2d(S, x, 0, std::min<long>((long)*ninit * *n, *lx));
+        jdhip_pjd_gep(c, *n, a, ja, ia, b, jb, ib, eigs, res, S.dX, *lx, neig, *sigma, *isearch,
+                      *ninit, *madspace, iter, *tol, shift, droptol, mem, icntl, *iprint, info,
+                      gap);
+        long cnt = (long)*n * std::max(1, std::min(nwant, *neig + 1));
+        copy_d2h(S, x, 0, std::min<long>(cnt, *lx));
+    } catch (const std::exception &e) {
+        set_err(c, e.what());
+        *info = JDHIP_INFO_DEVICE_ERROR;
+    }
+}
+
+void dpjdrevcom_gep_(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib,
+                     double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+                     int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                     double *shift, double *droptol, double *mem, int *icntl, int *ijob,
+                     int *ndx1, int *ndx2, int *ndx3, int *iprint, int *info, double *gap) {
+    static int nwant = 0;
+    if (*ijob == 0) nwant = *neig;
+    legacy_revcom_gep(*n, x, *lx, *ninit, nwant, ijob, ndx1, ndx2, ndx3, info, neig,
+                      [&](jdhip_ctx *c, double *dx, long *l1, long *l2, long *l3) {
+                          *l1 = *ndx1;
+                          *l2 = *ndx2;
+                          *l3 = *ndx3;
+                          jdhip_pjdrevcom_gep(c, *n, a, ja, ia, b, jb, ib, eigs, res, dx, *lx,
+                                              neig, *sigma, *isearch, *ninit, *madspace, iter,
+                                              *tol, shift, droptol, mem, icntl, ijob, l1, l2,
+                                              l3, *iprint, info, gap);
+                      });
+}
+
+void djdrevcom_gep_(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                    double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+                    double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
+                    int *info, double *gap) {
+    static int nwant = 0;
+    if (*ijob == 0) nwant = *neig;
+    legacy_revcom_gep(*n, x, *lx, *ninit, nwant, ijob, ndx1, ndx2, ndx3, info, neig,
+                      [&](jdhip_ctx *c, double *dx, long *l1, long *l2, long *l3) {
+                          *l1 = *ndx1;
+                          *l2 = *ndx2;
+                          *l3 = *ndx3;
+                          jdhip_jdrevcom_gep(c, *n, eigs, res, dx, *lx, neig, *sigma, *isearch,
+                                             *ninit, *madspace, iter, *tol, ijob, l1, l2, l3,
+                                             *iprint, info, gap);
+                      });
+}
+
+void dpjd_gep(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib, double *eigs,
+              double *res, double *x, int *lx, int *neig, double *sigma, int *isearch,
+              int *ninit, int *madspace, int *iter, double *tol, double *shift,
+              double *droptol, double *mem, int *icntl, int *iprint, int *info, double *gap) {
+    dpjd_gep_(n, a, ja, ia, b, jb, ib, eigs, res, x, lx, neig, sigma, isearch, ninit, madspace,
+              iter, tol, shift, droptol, mem, icntl, iprint, info, gap);
+}
+void dpjdrevcom_gep(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib,
+                    double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+                    int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                    double *shift, double *droptol, double *mem, int *icntl, int *ijob,
+                    int *ndx1, int *ndx2, int *ndx3, int *iprint, int *info, double *gap) {
+    dpjdrevcom_gep_(n, a, ja, ia, b, jb, ib, eigs, res, x, lx, neig, sigma, isearch, ninit,
+                    madspace, iter, tol, shift, droptol, mem, icntl, ijob, ndx1, ndx2, ndx3,
+                    iprint, info, gap);
+}
+void djdrevcom_gep(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                   double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+                   double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
+                   int *info, double *gap) {
+    djdrevcom_gep_(n, eigs, res, x, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
+                   ndx1, ndx2, ndx3, iprint, info, gap);
 }
 
 void dpjdcleanup_(void) {

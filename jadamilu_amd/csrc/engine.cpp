@@ -107,17 +107,29 @@ double JdEngine::dnorm(const double *w, long n) {
     return std::sqrt(v);
 }
 
-// REPGS (JDaux.F:2674-2753): iterated classical Gram-Schmidt of w against the
-// jv columns at voff.  y[0:jv) coefficients, y[jv] final norm (gam >= 0).
-void JdEngine::repgs(int n, long voff, int jv, long woff, std::vector<double> &y, int gam) {
+double JdEngine::ddot(const double *a, const double *b, long n) {
+    double *d = ctx_->slot(2);
+    dot(ctx_->stream, ctx_->rb, a, b, n, d);
+    double v;
+    fetch(&v, d, 1);
+    return v;
+}
+
+// REPGS (JDaux.F:2674-2753): iterated classical Gram-Schmidt of w, the
+// coefficients y = BV^T w from the jv columns at bvoff and the update
+// w -= V y with those at voff (V = BV for the standard problem).  y[0:jv)
+// coefficients, y[jv] final norm (gam >= 0).
+void JdEngine::repgs2(int n, long voff, long bvoff, int jv, long woff, std::vector<double> &y,
+                      int gam) {
     if (jv <= 0) return;
     if ((int)y.size() < 2 * jv + 2) y.resize(2 * jv + 2);
     hipStream_t s = ctx_->stream;
     const double *V = X_ + voff;
+    const double *BV = X_ + bvoff;
     double *w = X_ + woff;
     double *dy = ctx_->slot(0);
     if (gam < 0) {
-        multidot(s, ctx_->rb, V, n, jv, w, n, dy);
+        multidot(s, ctx_->rb, BV, n, jv, w, n, dy);
         gemv_n(s, ctx_->rb, w, V, n, jv, dy, -1.0, 1.0, n, nullptr);
         fetch(y.data(), dy, jv);
         return;
@@ -129,7 +141,7 @@ restart : {
     if (nrnd) std::fill(y.begin(), y.begin() + jv, 0.0);
     for (;;) {
         nro = nrn;
-        multidot(s, ctx_->rb, V, n, jv, w, n, dy);
+        multidot(s, ctx_->rb, BV, n, jv, w, n, dy);
         gemv_n(s, ctx_->rb, w, V, n, jv, dy, -1.0, 1.0, n, ctx_->slot(1));
         double buf[kMaxCols + 1];
         fetch(buf, dy, kMaxCols + 1);
@@ -166,7 +178,7 @@ fail:
 // SORTEV (JDaux.F:2464-2540): the permutation is computed on the host, the
 // column cycles are applied on the device.
 void JdEngine::sortev_dev(int n, int j, long voff, double *eigs, double *r, int it,
-                          double sigma) {
+                          double sigma, long v2off) {
     if (j <= 1) return;
     std::vector<double> key(j);
     std::vector<int> iw(j);
@@ -181,10 +193,14 @@ void JdEngine::sortev_dev(int n, int j, long voff, double *eigs, double *r, int 
         for (int i = j - 2; i >= 0; --i) iw[i + 1] = iw[i];
         iw[0] = is;
     }
-    ctx_->ensure_tmp((long)n);
+    // IT = -4 moves the columns of a second matrix (B X) along (JDaux.F:2511-2532)
+    const bool two = (it == -4) && v2off >= 0;
+    ctx_->ensure_tmp((long)n * (two ? 2 : 1));
     hipStream_t s = ctx_->stream;
     const size_t bytes = sizeof(double) * n;
     auto col = [&](int k) { return X_ + voff + (long)(k - 1) * n; };
+    auto col2 = [&](int k) { return X_ + v2off + (long)(k - 1) * n; };
+    double *tmp2 = ctx_->d_tmp + n;
     int ic = 0;
     for (;;) {
         int is = 0;
@@ -200,10 +216,13 @@ void JdEngine::sortev_dev(int n, int j, long voff, double *eigs, double *r, int 
         }
         int ir = iw[is - 1];
         JDHIP_CHECK(hipMemcpyAsync(ctx_->d_tmp, col(is), bytes, hipMemcpyDeviceToDevice, s));
+        if (two) JDHIP_CHECK(hipMemcpyAsync(tmp2, col2(is), bytes, hipMemcpyDeviceToDevice, s));
         double r1 = 0.0, w1 = eigs[is - 1];
         if (it == 4 || it == -4) r1 = r[is - 1];
         for (;;) {
             JDHIP_CHECK(hipMemcpyAsync(col(is), col(ir), bytes, hipMemcpyDeviceToDevice, s));
+            if (two)
+                JDHIP_CHECK(hipMemcpyAsync(col2(is), col2(ir), bytes, hipMemcpyDeviceToDevice, s));
             if (it == 4 || it == -4) r[is - 1] = r[ir - 1];
             eigs[is - 1] = eigs[ir - 1];
             iw[is - 1] = -ir;
@@ -213,6 +232,7 @@ void JdEngine::sortev_dev(int n, int j, long voff, double *eigs, double *r, int 
             if (!(iw[ir - 1] > 0)) break;
         }
         JDHIP_CHECK(hipMemcpyAsync(col(is), ctx_->d_tmp, bytes, hipMemcpyDeviceToDevice, s));
+        if (two) JDHIP_CHECK(hipMemcpyAsync(col2(is), tmp2, bytes, hipMemcpyDeviceToDevice, s));
         if (it == 4 || it == -4) r[is - 1] = r1;
         eigs[is - 1] = w1;
         iw[is - 1] = -ir;
@@ -282,6 +302,11 @@ int JdEngine::shrink(int n, int j, int k, int ucol_off, int iserch) {
     tall_gemm_mfma(s, X_ + ww, n, k, dq, k, j, ctx_->d_tmp, n, n);
     JDHIP_CHECK(hipMemcpyAsync(X_ + ww, ctx_->d_tmp, sizeof(double) * n * j,
                                hipMemcpyDeviceToDevice, s));
+    if (gep_) {  // BV := BV Q as well (JDaux.F:2439)
+        tall_gemm_mfma(s, X_ + bv, n, k, dq, k, j, ctx_->d_tmp, n, n);
+        JDHIP_CHECK(hipMemcpyAsync(X_ + bv, ctx_->d_tmp, sizeof(double) * n * j,
+                                   hipMemcpyDeviceToDevice, s));
+    }
     const int ld = modspace;
     for (int a = 1; a < k; ++a)
         for (int b = 0; b < a; ++b) MM[a + b * ld] = MM[b + a * ld];
@@ -385,6 +410,75 @@ L40:
     *theta_ = st_bk;
 }
 
+// STOPTEST with the terms of the generalized problem (JDaux.F:2590-2668):
+// the residual norm is scaled by ||B u||, the Rayleigh-quotient estimate
+// carries the B-corrections pcad (accumulated BU.Q alpha) and pcr (BU.r at
+// the start), and the norm of the correction is measured through BT = B x.
+void JdEngine::stoptest_gep(int n, double tol, double *rk, double *alprho_, double gk_,
+                            double *theta_, double eta_, int *info, double trs_, double pcad,
+                            double pcr) {
+    if (*info == 0) goto L10;
+    st_istag = 0;
+    st_istt = 0;
+    st_g0 = *rk;
+    st_gkm1 = *rk;
+    st_gtrs = trs_ * st_g0;
+    if (st_gtrs < tol) st_gtrs = tol;
+    st_gs = st_gtrs;
+    st_gkmin = st_g0;
+    st_bk = *theta_ - eta_;
+    st_bkgep = *theta_ - eta_ + pcr;
+    st_un2 = -1.0;
+L10:
+    st_bk = st_bk - *alprho_;
+    st_bkgep = st_bkgep - pcad;
+    st_beta = std::fabs(st_bkgep);
+    gk_ = gk_ / nbu;
+    if (gk_ < st_gkmin) {
+        st_gkmin = gk_;
+        st_istag = 0;
+    }
+    if (gk_ > st_gkm1) {
+        ++st_istag;
+        ++st_istt;
+    }
+    if (st_istag >= 25 || st_istt >= 100) goto L40;
+    if (st_gkmin >= st_gtrs) goto L20;
+    st_toupdate = true;
+    if (gk_ >= st_gs) goto L15;
+L13 : {
+    const double tnb = dnorm(X_ + bt, n) / nbu;
+    const double bubt = -ddot(X_ + bu, X_ + bt, n) / (nbu * nbu);
+    st_alp = std::fabs(1.0 + bubt);
+    st_tn = std::sqrt((tnb + bubt) * (tnb - bubt)) / st_alp;
+}
+    st_un2 = 1.0 + st_tn * st_tn;
+    st_toupdate = false;
+    st_gs = (gk_ < 0.1 * st_g0) ? tol : 0.1 * st_g0;
+    if (st_gs < tol) st_gs = tol;
+L15:
+    if (st_beta < gk_ * st_tn)
+        *rk = std::sqrt((gk_ * gk_ + st_beta * st_beta) / st_un2) / st_alp;
+    else
+        *rk = (gk_ + st_beta * st_tn) / (st_alp * st_un2);
+    if (*rk < tol && st_toupdate) goto L13;
+    if (*rk < tol) goto L40;
+    if (gk_ * std::sqrt(st_un2) < 15 * st_tn * st_beta &&
+        st_beta * st_tn > 0.5 * tol * st_un2 * st_alp)
+        goto L40;
+    if (st_istag >= 3) goto L40;
+    if (st_istag >= 1 && st_beta * st_tn > 0.5 * tol * st_un2 * st_alp && gk_ < st_gtrs)
+        goto L40;
+L20:
+    *info = 0;
+    st_gkm1 = gk_;
+    return;
+L40:
+    *info = std::max(*info, 1);
+    *alprho_ = st_un2 * st_alp * st_alp;
+    *theta_ = st_bk;
+}
+
 // ---------------------------------------------------------------------------
 // Progress line on stderr at most every 30 s of a solve when JDHIP_VERBOSE
 // is set (a solve of a large problem can run for minutes without returning).
@@ -408,7 +502,8 @@ void JdEngine::progress(int matvecs, int converged) {
 
 void JdEngine::step(int n, double *eigs, double *res, double *x, long lx, int *neig,
                     double sigma, int isearch, int ninit, int madspace, int *iter, double tol,
-                    int *ijob, long *ndx1, long *ndx2, int iprint, int *info, double *gap) {
+                    int *ijob, long *ndx1, long *ndx2, long *ndx3, int iprint, int *info,
+                    double *gap, bool gep) {
     (void)iprint;
     X_ = x;
     hipStream_t s = ctx_->stream;
@@ -418,6 +513,7 @@ void JdEngine::step(int n, double *eigs, double *res, double *x, long lx, int *n
 
     if (*ijob == 0) {
         progress_started_ = false;
+        gep_ = gep;
         goto L1;
     }
     switch (rlbl) {
@@ -427,6 +523,10 @@ void JdEngine::step(int n, double *eigs, double *res, double *x, long lx, int *n
     case 4: goto L38;
     case 5: goto L255;
     case 6: goto L55;
+    case 11: goto L15;
+    case 12: goto L45;
+    case 13: goto LG38;
+    case 14: goto LG37;
     default: break;
     }
     *info = -15;
@@ -446,6 +546,11 @@ L1:
     else if ((long)n * (2 * madspace + *neig + 4) + 3L * madspace * madspace +
                  std::max(madspace * madspace, *neig) > lx)
         *info = -10;
+    else if (gep_ && (long)n * (3 * madspace + 2 * *neig + 4) + 3L * madspace * madspace +
+                         std::max(madspace * madspace, *neig) > lx)
+        *info = -10;  // JDaux.F:1512-1515
+    else if (gep_ && isearch >= 2)
+        *info = -8;  // harmonic extraction of the generalized problem: not supported
     else if (*iter <= 0) *info = -11;
     else if (tol <= 0.0) *info = -12;
     else if (std::min(*neig, n) + std::min(madspace, n) > kMaxCols)
@@ -460,12 +565,20 @@ L1:
         xx = 0;
         vv = xx + (long)nwant * N;
         ww = vv + MS * N;
-        bx = xx;
-        bv = vv;
+        long mm_off;
+        if (gep_) {  // JDaux.F:1537-1544
+            bx = ww + MS * N;
+            bv = bx + (long)nwant * N;
+            mm_off = bv + MS * N;
+        } else {
+            bx = xx;
+            bv = vv;
+            mm_off = ww + MS * N;
+        }
         // MM, NN, MP, NP live on the host; WCG keeps the reference offset
-        long mm_off = ww + MS * N;
         wcg = mm_off + 3 * MS * MS + std::max(MS * MS, (long)nwant);
         xcg2 = ww + (MS - 1) * N;
+        bt = bv + (MS - 1) * N;
         reig = wcg;
         uu = xx + (long)(nwant - 1) * N;
         bu = bx + (long)(nwant - 1) * N;
@@ -527,21 +640,34 @@ L2:
     IJ = 0;
 L4:
     ++IJ;
-    repgs(n, kxp, *neig + IJ - 1, vv + (long)(IJ - 1) * n, ycoef, 1);
-    if (*neig + IJ - 1 > 0) sclr1 = ycoef[*neig + IJ - 1];
-    else sclr1 = dnorm(D(vv), n);
-    scal(s, D(vv + (long)(IJ - 1) * n), 1.0 / sclr1, n);
+    repgs2(n, kxp, kbxp, *neig + IJ - 1, vv + (long)(IJ - 1) * n, ycoef, 1);
+    if (!gep_) {
+        if (*neig + IJ - 1 > 0) sclr1 = ycoef[*neig + IJ - 1];
+        else sclr1 = dnorm(D(vv), n);
+        scal(s, D(vv + (long)(IJ - 1) * n), 1.0 / sclr1, n);
+    }
     *ndx1 = vv + (long)(IJ - 1) * n;
     *ndx2 = ww + (long)(IJ - 1) * n;
+    *ndx3 = bv + (long)(IJ - 1) * n;
     rlbl = 1;
     if (internal_mv()) {
         do_matvec(D(*ndx1), D(*ndx2));
+        if (gep_) do_bmatvec(D(*ndx1), D(*ndx3));
         goto L5;
     }
     *ijob = 1;
     return;
 L5:
     ++*iter;
+    if (gep_) {  // B-normalise v, B v and A v (JDaux.F:1637-1644)
+        const long c = (long)(IJ - 1) * n;
+        sclr1 = ddot(D(vv + c), D(bv + c), n);
+        if (sclr1 <= 0.0) goto L196;
+        sclr1 = std::sqrt(sclr1);
+        scal(s, D(bv + c), 1.0 / sclr1, n);
+        scal(s, D(vv + c), 1.0 / sclr1, n);
+        scal(s, D(ww + c), 1.0 / sclr1, n);
+    }
     if (IJ < J) goto L4;
     axpyalpha = -sigma;
     gram_initial(n);
@@ -560,12 +686,15 @@ L10:  // extraction
         put(ctx_->slot(3), YY(), J);
         gemv_n(s, rb, D(uu), D(vv), n, J, ctx_->slot(3), 1.0, 0.0, n, nullptr);
         gemv_n(s, rb, D(reig), D(ww), n, J, ctx_->slot(3), 1.0, 0.0, n, nullptr);
+        if (gep_) gemv_n(s, rb, D(bu), D(bv), n, J, ctx_->slot(3), 1.0, 0.0, n, nullptr);
     } else {
         YY()[0] = 1.0;
         for (int c = 1; c < J; ++c) YY()[c] = 0.0;
         JDHIP_CHECK(hipMemcpyAsync(D(uu), D(vv), nb, hipMemcpyDeviceToDevice, s));
         JDHIP_CHECK(hipMemcpyAsync(D(reig), D(ww), nb, hipMemcpyDeviceToDevice, s));
+        if (gep_) JDHIP_CHECK(hipMemcpyAsync(D(bu), D(bv), nb, hipMemcpyDeviceToDevice, s));
     }
+    if (gep_) nbu = dnorm(D(bu), n);
     if (iserch == 2) {
         symv_upper(J, MM.data(), modspace, YY(), WL.data());
         thmt = small_dot(J, WL.data(), YY());
@@ -582,6 +711,7 @@ L10:  // extraction
         fetch(&v, d, 1);
         nreigp = nreig;
         nreig = std::sqrt(v);
+        if (gep_) nreig = nreig / nbu;
     }
     if (neweig) {
         rinit = nreig;
@@ -593,9 +723,27 @@ L10:  // extraction
     if (nreig >= tol && rndind <= 1.0) goto L20;
     if (nreig >= tol && resid > 0.1 * nreig) goto L20;
     // convergence detected
-    repgs(n, kxp, *neig, uu, ycoef, 0);
+    repgs2(n, kxp, kbxp, *neig, uu, ycoef, 0);
     if (*neig > 0 && ycoef[*neig] == 0.0) goto L195;
+    if (gep_) {  // B u of the orthogonalised u, then B-normalise (labels 14-15)
+        *ndx1 = uu;
+        *ndx3 = bu;
+        rlbl = 11;
+        if (internal_mv()) {
+            do_bmatvec(D(*ndx1), D(*ndx3));
+            goto L15;
+        }
+        *ijob = 4;
+        return;
+    }
     sclr1 = (*neig > 0) ? ycoef[*neig] : dnorm(D(uu), n);
+    goto L16;
+L15:
+    sclr1 = ddot(D(uu), D(bu), n);
+    if (sclr1 <= 0.0) goto L196;
+    sclr1 = std::sqrt(sclr1);
+    scal(s, D(bu), 1.0 / sclr1, n);
+L16:
     scal(s, D(uu), 1.0 / sclr1, n);
 
 L18:  // lock the converged pair
@@ -682,7 +830,7 @@ L20:  // expansion
         }
         if (nreig < std::fabs(delta1) && std::fabs(delta1 / deltao - 1.0) < 0.1) eta = theta;
     }
-    repgs(n, bu, *neig + 1, reig, ycoef, -1);
+    repgs2(n, bu, uu, *neig + 1, reig, ycoef, -1);
     rndind = std::fabs(ycoef[*neig] / (nreig * epsmsr));
     resid = nreig;
     xcg = vv + (long)J * n;
@@ -694,6 +842,10 @@ L20:  // expansion
     if (*neig <= 0) {
         shift1 = theta;
         *gap = (J > 1) ? theta2 - theta : 0.0;
+    }
+    if (gep_) {
+        axpybeta = ddot(D(bu), D(reig), n) / (nbu * nbu);
+        goto LG0;
     }
 
     // ---- LSOLREVCOM init (JDaux.F:3086-3116) ----
@@ -828,6 +980,111 @@ L40i:  // inner iteration fails (JDaux.F:3255-3266)
             JDHIP_CHECK(
                 hipMemcpyAsync(D(xcg), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
     }
+    goto L34;
+    // ---- inner solver of the generalized problem: LSOLREVCOM with the GEP
+    // terms of JDRVCOM labels 30-38 (JDaux.F:1883-1934, 2989-3268), scalars on
+    // the host.  Z holds B p after each matvec (NDX3 = Z) and BT accumulates
+    // alpha B p.
+LG0:
+    ls_posdef = (ifo == 1);
+    ifo = 0;
+    ls_maxit = itcg;
+    ls_iter = 0;
+    f32_ = false;
+    if (ls_posdef) fill(s, D(xcg), 0.0, n);
+    fill(s, D(bt), 0.0, n);
+    g_ggk2 = gk * gk;
+LG10:
+    ++ls_iter;
+    ++inner_iters;
+    *ndx1 = wcg + n;  // Z
+    *ndx2 = wcg;      // R
+    rlbl = 13;
+    if (internal_pc()) {
+        ops_->precond(*this, D(*ndx1), D(*ndx2));
+        goto LG38;
+    }
+    *ijob = 2;
+    return;
+LG38:
+    flagcg = true;
+    repgs2(n, uu, bu, *neig + 1, wcg + n, ycoef, -1);  // z -= U (BU^T z)
+    g_rho = ddot(D(wcg), D(wcg + n), n);
+    if ((ls_posdef && g_rho <= 0.0) || g_rho == 0.0) goto LG40;
+    if (ls_iter > 1) {
+        axpy(s, D(wcg + n), g_rho / g_rho1, D(wcg + 2L * n), n);
+        JDHIP_CHECK(hipMemcpyAsync(D(wcg + 2L * n), D(wcg + n), nb, hipMemcpyDeviceToDevice, s));
+    } else {
+        JDHIP_CHECK(hipMemcpyAsync(D(wcg + 2L * n), D(wcg + n), nb, hipMemcpyDeviceToDevice, s));
+    }
+    *ndx1 = wcg + 2L * n;  // P
+    *ndx2 = wcg + 3L * n;  // Q = A P
+    *ndx3 = wcg + n;       // Z = B P
+    rlbl = 14;
+    if (internal_mv()) {
+        do_matvec(D(*ndx1), D(*ndx2));
+        do_bmatvec(D(*ndx1), D(*ndx3));
+        goto LG37;
+    }
+    *ijob = 1;
+    return;
+LG37:
+    axpy(s, D(wcg + 3L * n), -eta, D(wcg + n), n);  // Q = (A - eta B) P
+    axpyalpha = ddot(D(bu), D(wcg + 3L * n), n) / (nbu * nbu);
+    repgs2(n, bu, uu, 1, wcg + 3L * n, ycoef, -1);  // Q -= BU (U^T Q)
+    alprho = ycoef[0];
+    ++*iter;
+    {  // LSOLREVCOM label 4
+        const double pq = ddot(D(wcg + 2L * n), D(wcg + 3L * n), n);
+        alpha = pq;
+        if ((ls_posdef && pq <= 0.0) || pq == 0.0) goto LG40;
+        alpha = g_rho / pq;
+    }
+    {
+        double *d = ctx_->slot(2), v;
+        axpy_nrm2(s, rb, D(wcg), -alpha, D(wcg + 3L * n), n, d);  // R -= alpha Q
+        fetch(&v, d, 1);
+        gk = std::sqrt(v);
+    }
+    axpy(s, D(bt), alpha, D(wcg + n), n);  // BT += alpha B P
+    if (ls_posdef) {
+        axpy(s, D(xcg), alpha, D(wcg + 2L * n), n);
+    } else {
+        const double gk2 = gk * gk;
+        g_ttk2m = g_ttk2;
+        g_ttk2 = gk2 / g_ggk2;
+        const double ck2 = 1.0 / (1.0 + g_ttk2);
+        g_ggk2 = g_ggk2 * g_ttk2 * ck2;
+        if (ls_iter > 1) {
+            axpy(s, D(xcg2), alpha / g_ttk2m, D(wcg + 2L * n), n);
+            scal(s, D(xcg2), g_ttk2m * ck2, n);
+            axpy(s, D(xcg), 1.0, D(xcg2), n);
+        } else {
+            JDHIP_CHECK(
+                hipMemcpyAsync(D(xcg2), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
+            scal(s, D(xcg2), alpha * ck2, n);
+            JDHIP_CHECK(hipMemcpyAsync(D(xcg), D(xcg2), nb, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    // L33: stopping test
+    if (ftflg) ifo = -1;
+    if (ftflg && iserch == 2) ifo = -2;
+    axpyalpha = axpyalpha * alpha;
+    alprho = alprho * alpha;
+    stoptest_gep(n, tol, &resid, &alprho, gk, &theta, eta, &ifo, trs, axpyalpha, axpybeta);
+    ftflg = false;
+    if (ifo == 1) {
+        ifo = 0;
+        goto L34;
+    }
+    if (ls_iter == ls_maxit) goto LG40;
+    g_rho1 = g_rho;
+    goto LG10;
+LG40:  // inner iteration fails
+    ifo = 1;
+    if (alpha <= 0.0) ifo = 2;
+    if (ls_iter == 1)
+        JDHIP_CHECK(hipMemcpyAsync(D(xcg), D(wcg + 2L * n), nb, hipMemcpyDeviceToDevice, s));
 L34:
     f32_last_ = f32_;
     if (f32_) {
@@ -845,30 +1102,61 @@ L34:
     if (resid >= tol || f32_last_) goto L50;
     // convergence reached during the inner solve
     axpy(s, D(uu), -1.0, D(xcg), n);
-    repgs(n, kxp, *neig, uu, ycoef, 0);
+    repgs2(n, kxp, kbxp, *neig, uu, ycoef, 0);
     if (*neig > 0 && ycoef[*neig] == 0.0) goto L195;
+    if (gep_) {  // B u, then B-normalise (labels 44-45)
+        *ndx1 = uu;
+        *ndx3 = bu;
+        rlbl = 12;
+        if (internal_mv()) {
+            do_bmatvec(D(*ndx1), D(*ndx3));
+            goto L45;
+        }
+        *ijob = 4;
+        return;
+    }
     sclr1 = (*neig > 0) ? ycoef[*neig] : dnorm(D(uu), n);
+    goto L46;
+L45:
+    sclr1 = ddot(D(uu), D(bu), n);
+    if (sclr1 <= 0.0) goto L196;
+    sclr1 = std::sqrt(sclr1);
+    scal(s, D(bu), 1.0 / sclr1, n);
+L46:
     scal(s, D(uu), 1.0 / sclr1, n);
     theta = theta / sclr1 + eta;
     conv = true;
     nreig = resid;
     if (*neig >= nwant - 1) goto L18;
 L50:  // expand the subspace with the correction
-    repgs(n, kxp, *neig + J, xcg, ycoef, 1);
+    repgs2(n, kxp, kbxp, *neig + J, xcg, ycoef, 1);
     for (int c = 0; c < J; ++c) YY()[c] += -1.0 * ycoef[*neig + c];
-    sclr1 = ycoef[*neig + J];
-    scal(s, D(xcg), 1.0 / sclr1, n);
+    if (!gep_) {
+        sclr1 = ycoef[*neig + J];
+        scal(s, D(xcg), 1.0 / sclr1, n);
+    }
     *ndx1 = xcg;
     *ndx2 = ww + (long)J * n;
+    *ndx3 = bv + (long)J * n;
     rlbl = 6;
     if (internal_mv()) {
         do_matvec(D(*ndx1), D(*ndx2));
+        if (gep_) do_bmatvec(D(*ndx1), D(*ndx3));
         goto L55;
     }
     *ijob = 1;
     return;
 L55:
     ++*iter;
+    if (gep_) {  // B-normalise the new column (JDaux.F:2000-2008)
+        const long c = (long)J * n;
+        sclr1 = ddot(D(xcg), D(bv + c), n);
+        if (sclr1 <= 0.0) goto L196;
+        sclr1 = std::sqrt(sclr1);
+        scal(s, D(xcg), 1.0 / sclr1, n);
+        scal(s, D(ww + c), 1.0 / sclr1, n);
+        scal(s, D(bv + c), 1.0 / sclr1, n);
+    }
     YY()[J] = -sclr1;
     axpyalpha = -sigma;
     if (iserch == 2) axpy(s, D(*ndx2), axpyalpha, D(bv + (long)J * n), n);
@@ -931,11 +1219,16 @@ L193:
 L195:
     *info = -54320;
     goto L198;
+L196:  // the mass matrix B is not positive definite
+    *info = -54322;
+    goto L198;
 L197:
     *info = *neig + 1;
     rinitm *= rinit;
 L198:
     if (*neig > 0) JDHIP_CHECK(hipMemcpyAsync(D(vv), D(uu), nb, hipMemcpyDeviceToDevice, s));
+    if (*neig > 0 && gep_)
+        JDHIP_CHECK(hipMemcpyAsync(D(bv), D(bu), nb, hipMemcpyDeviceToDevice, s));
 L200:  // normal termination
     if (*neig < nwant) {
         rinit = std::pow(rinitm, 1.0 / (double)(*neig + 1));
@@ -947,11 +1240,18 @@ L200:  // normal termination
             std::memmove(res, res + nwant - JJ, sizeof(double) * JJ);
             JDHIP_CHECK(hipMemcpyAsync(D((long)n * (*neig) + xx), D(vv), nb,
                                        hipMemcpyDeviceToDevice, s));
+            if (gep_) {
+                JDHIP_CHECK(hipMemcpyAsync(D(bx), D((long)n * (nwant - JJ) + bx), nb * JJ,
+                                           hipMemcpyDeviceToDevice, s));
+                JDHIP_CHECK(hipMemcpyAsync(D((long)n * (*neig) + bx), D(bv), nb,
+                                           hipMemcpyDeviceToDevice, s));
+            }
             eigs[*neig] = theta;
             res[*neig] = nreig;
             theta2 = theta;
         } else {
             JDHIP_CHECK(hipMemcpyAsync(D(xx), D(uu), nb, hipMemcpyDeviceToDevice, s));
+            if (gep_) JDHIP_CHECK(hipMemcpyAsync(D(bx), D(bu), nb, hipMemcpyDeviceToDevice, s));
             eigs[0] = theta;
             res[0] = nreig;
             *gap = 0.0;
@@ -960,7 +1260,7 @@ L200:  // normal termination
     } else {
         rinit = std::pow(rinitm, 1.0 / (double)(*neig));
     }
-    sortev_dev(n, *neig, xx, eigs, res, 4, 0.0);
+    sortev_dev(n, *neig, xx, eigs, res, gep_ ? -4 : 4, 0.0, bx);
     if (iserch >= 2) {
         for (itarg = 1; itarg <= *neig; ++itarg)
             if (eigs[itarg - 1] > sigma) break;
@@ -1035,6 +1335,11 @@ L225:
                                    hipMemcpyDeviceToDevice, s));
         tall_gemm_mfma(s, D(ww), n, J, dq, J, J, ctx_->d_tmp, n, n);
         JDHIP_CHECK(hipMemcpyAsync(D(vv), ctx_->d_tmp, nb * J, hipMemcpyDeviceToDevice, s));
+        if (gep_) {  // BX of the cluster := BX Q (JDaux.F:2296-2300)
+            tall_gemm_mfma(s, D((long)n * (I - 1) + bx), n, J, dq, J, J, ctx_->d_tmp, n, n);
+            JDHIP_CHECK(hipMemcpyAsync(D((long)n * (I - 1) + bx), ctx_->d_tmp, nb * J,
+                                       hipMemcpyDeviceToDevice, s));
+        }
     }
     nrc = 0.0;
     gl = gl - eigs[I - 1];
@@ -1047,6 +1352,7 @@ L225:
                   ctx_->slot(2));
         fetch(&v, ctx_->slot(2), 1);
         nreig = std::sqrt(v);
+        if (gep_) nreig = nreig / dnorm(D((long)n * (I + IJ - 2) + bx), n);
         res[I + IJ - 2] = nreig;
         nrc += nreig * nreig;
     }
@@ -1075,6 +1381,7 @@ L255:
         axpy_nrm2(s, rb, D(vv), -theta, D((long)n * (I - 1) + bx), n, ctx_->slot(2));
         fetch(&v, ctx_->slot(2), 1);
         nreig = std::sqrt(v);
+        if (gep_) nreig = nreig / dnorm(D((long)n * (I - 1) + bx), n);
     }
     eigs[I - 1] = theta;
     res[I - 1] = nreig;

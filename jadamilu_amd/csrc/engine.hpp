@@ -71,6 +71,8 @@ struct Ops {
     virtual bool has_f32() const { return false; }
     virtual void matvec_f(const float *x, float *y, double eta, const float *u, double *udot) {}
     virtual void precond_f(JdEngine &e, float *z, const float *r) {}
+    // generalized problem A x = lambda B x: y = B x (inline when has_matvec)
+    virtual void bmatvec(const double *x, double *y) {}
 };
 
 class JdEngine {
@@ -81,12 +83,17 @@ class JdEngine {
     }
 
     // One JDRVCOM call.  x is the DEVICE workspace (lx doubles), eigs/res
-    // host arrays.  On return *ijob: 0 done, 1 matvec X(ndx2)=A X(ndx1),
-    // 2 precond X(ndx1)=P^-1 X(ndx2), 3 matvec (final checks).  ndx are
-    // 0-based offsets into x.
+    // host arrays.  On return *ijob: 0 done, 1 matvec X(ndx2)=A X(ndx1) and,
+    // for the generalized problem, X(ndx3)=B X(ndx1), 2 precond
+    // X(ndx1)=P^-1 X(ndx2), 3 matvec X(ndx2)=A X(ndx1) only (final checks),
+    // 4 X(ndx3)=B X(ndx1) only.  ndx are 0-based offsets into x.  gep (read
+    // when *ijob = 0): A x = lambda B x with B symmetric positive definite
+    // (JDRVCOM's GEP branches), ISEARCH <= 1.
     void step(int n, double *eigs, double *res, double *x, long lx, int *neig, double sigma,
               int isearch, int ninit, int madspace, int *iter, double tol, int *ijob,
-              long *ndx1, long *ndx2, int iprint, int *info, double *gap);
+              long *ndx1, long *ndx2, long *ndx3, int iprint, int *info, double *gap,
+              bool gep = false);
+    bool gep() const { return gep_; }
 
     void set_ops(Ops *ops) { ops_ = ops; }
 
@@ -99,8 +106,15 @@ class JdEngine {
   private:
     // ---- device helpers ----
     double dnorm(const double *w, long n);
-    void repgs(int n, long voff, int jv, long woff, std::vector<double> &y, int gam);
-    void sortev_dev(int n, int j, long voff, double *eigs, double *r, int it, double sigma);
+    void repgs(int n, long voff, int jv, long woff, std::vector<double> &y, int gam) {
+        repgs2(n, voff, voff, jv, woff, y, gam);
+    }
+    // REPGS with separate bases: coefficients from the columns at bvoff,
+    // subtraction with those at voff (the same for the standard problem)
+    void repgs2(int n, long voff, long bvoff, int jv, long woff, std::vector<double> &y,
+                int gam);
+    void sortev_dev(int n, int j, long voff, double *eigs, double *r, int it, double sigma,
+                    long v2off = -1);
     int solprojep(int j, double *mm, double *nn, double *theta, double *theta2,
                   std::vector<double> &w, int is);
     int shrink(int n, int j, int k, int ucol_off, int iserch);
@@ -110,6 +124,12 @@ class JdEngine {
     void put(double *dev, const double *host, long cnt);
     void stoptest(int n, double tol, double *rk, double alprho, double gk, double *theta,
                   double eta, int *info, double trs, double xn2);
+    // generalized problem: the GEP terms of STOPTEST (pcad, pcr, ||BT||,
+    // BU.BT); may change alprho
+    void stoptest_gep(int n, double tol, double *rk, double *alprho, double gk, double *theta,
+                      double eta, int *info, double trs, double pcad, double pcr);
+    void do_bmatvec(const double *xin, double *yout) { ops_->bmatvec(xin, yout); }
+    double ddot(const double *a, const double *b, long n);
     bool internal_mv() const { return ops_ && ops_->has_matvec(); }
     bool internal_pc() const { return ops_ && ops_->has_precond(); }
     bool use_f32() const { return internal_mv() && internal_pc() && ops_->has_f32(); }
@@ -137,7 +157,11 @@ class JdEngine {
     bool ftflg = false, conv = false, uknown = false, initflg = false, neweig = false;
     int mvmax = 0, midspace = 0, modspace = 0, nwant = 0, itarg = 0, iserch = 0;
     long xx = 0, xcg = 0, xcg2 = 0, wcg = 0, vv = 0, ww = 0, uu = 0, reig = 0;
-    long bx = 0, bu = 0, bv = 0, kxp = 0, kbxp = 0;
+    long bx = 0, bu = 0, bv = 0, kxp = 0, kbxp = 0, bt = 0;
+    bool gep_ = false;
+    double nbu = 1.0, axpybeta = 0.0;
+    // inner solver of the generalized problem (host scalars)
+    double g_rho = 0, g_rho1 = 0, g_ttk2 = 0, g_ttk2m = 0, g_ggk2 = 0;
     int iterp = 0, itcg = 0, ifo = 0, istep = 0, istag = 0;
     int I = 0, IJ = 0, J = 0, K = 0, II = 0, JJ = 0;
     double resid = 0, sclr1 = 0, sclr2 = 0, theta = 0, eta = 0, thmt = 0, delta1 = 0,
@@ -154,7 +178,7 @@ class JdEngine {
     int st_istag = 0, st_istt = 0;
     bool st_toupdate = false;
     double st_g0 = 0, st_gs = 0, st_gtrs = 0, st_gkmin = 0, st_bk = 0, st_beta = 0,
-           st_tn = 0, st_un2 = 0, st_gkm1 = 0, st_alp = 0;
+           st_tn = 0, st_un2 = 0, st_gkm1 = 0, st_alp = 0, st_bkgep = 0;
 
     unsigned long long rng_ = 0x2545F4914F6CDD1DULL;
 

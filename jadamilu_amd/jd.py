@@ -24,6 +24,12 @@ def lx_optimal(n, neig, madspace):
     return n * (3 * madspace + neig + 1) + 3 * madspace ** 2 + max(madspace ** 2, neig)
 
 
+def lx_optimal_gep(n, neig, madspace):
+    """Workspace of the generalized drivers, N*(4*MADSPACE+2*NEIG+1)+3*MADSPACE**2+
+    MAX(MADSPACE**2,NEIG) (PJD.F:544-545)."""
+    return n * (4 * madspace + 2 * neig + 1) + 3 * madspace ** 2 + max(madspace ** 2, neig)
+
+
 @dataclass
 class JDResult:
     eigs: np.ndarray
@@ -145,6 +151,82 @@ def pjd_host(n, a, ja, ia, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20, 
     return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(),
                     x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=IT.value,
                     info=INFO.value, gap=GAP.value, shift=SH.value, droptol=DT.value)
+
+
+def pjd_gep_host(n, a, ja, ia, b, jb, ib, *, neig, isearch=0, sigma=0.0, ninit=0, madspace=20,
+                 maxiter=1000, tol=1e-10, shift=0.0, droptol=1e-3, mem=20.0,
+                 icntl=(0, 0, 0, 0, 0), x0=None, eigs0=None, lx=None, iprint=0):
+    """DPJD_GEP through the legacy symbol `dpjd_gep_`: A x = lambda B x."""
+    L = lib()
+    lx = lx or lx_optimal_gep(n, neig, madspace)
+    x = np.zeros(lx)
+    if x0 is not None:
+        x[: np.size(x0)] = np.asarray(x0).ravel(order="F")
+    a, b = np.array(a, dtype=np.float64), np.array(b, dtype=np.float64)
+    ja, jb = np.array(ja, dtype=np.int32), np.array(jb, dtype=np.int32)
+    ia, ib = np.array(ia, dtype=np.int32), np.array(ib, dtype=np.int32)
+    eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+    res = np.zeros(max(neig, 1))
+    ic = _icntl(icntl)
+    N, LX, NE, IS, NI, MS, IT = _i(n), _i(lx), _i(neig), _i(isearch), _i(ninit), _i(madspace), _i(maxiter)
+    SG, TL, SH, DT, ME = _d(sigma), _d(tol), _d(shift), _d(droptol), _d(mem)
+    IP, INFO, GAP = _i(iprint), _i(0), _d(0)
+    L.dpjd_gep_(C.byref(N), _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(b), _p(jb, C.c_int),
+                _p(ib, C.c_int), _p(eigs), _p(res), _p(x), C.byref(LX), C.byref(NE),
+                C.byref(SG), C.byref(IS), C.byref(NI), C.byref(MS), C.byref(IT), C.byref(TL),
+                C.byref(SH), C.byref(DT), C.byref(ME), _p(ic, C.c_int), C.byref(IP),
+                C.byref(INFO), C.byref(GAP))
+    k = NE.value if INFO.value >= 0 else 0
+    return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(),
+                    x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=IT.value,
+                    info=INFO.value, gap=GAP.value, shift=SH.value, droptol=DT.value)
+
+
+def pjdrevcom_gep_host(n, a, ja, ia, b, jb, ib, matvec_a, matvec_b, *, neig, isearch=0,
+                       sigma=0.0, ninit=0, madspace=20, maxiter=1000, tol=1e-10, shift=0.0,
+                       droptol=1e-3, mem=20.0, icntl=(0, 0, 0, 0, 0), lx=None, iprint=0):
+    """DPJDREVCOM_GEP through `dpjdrevcom_gep_`; matvec_a(x, y) and
+    matvec_b(x, y) write A x and B x into y (host numpy arrays).  Diagonal
+    preconditioning: a = diagonal of A with ja = ia = [-1] (and b, jb, ib
+    likewise), as in samples/EXAMPLE9_GEP.f."""
+    L = lib()
+    lx = lx or lx_optimal_gep(n, neig, madspace)
+    x = np.zeros(lx)
+    a, b = np.array(a, dtype=np.float64), np.array(b, dtype=np.float64)
+    ja, jb = np.array(ja, dtype=np.int32), np.array(jb, dtype=np.int32)
+    ia, ib = np.array(ia, dtype=np.int32), np.array(ib, dtype=np.int32)
+    eigs, res = np.zeros(max(neig, 1)), np.zeros(max(neig, 1))
+    ic = _icntl(icntl)
+    N, LX, NE, IS, NI, MS, IT = _i(n), _i(lx), _i(neig), _i(isearch), _i(ninit), _i(madspace), _i(maxiter)
+    SG, TL, SH, DT, ME = _d(sigma), _d(tol), _d(shift), _d(droptol), _d(mem)
+    IJ, N1, N2, N3, IP, INFO, GAP = _i(0), _i(0), _i(0), _i(0), _i(iprint), _i(0), _d(0)
+    counts = {1: 0, 3: 0, 4: 0}
+    while True:
+        L.dpjdrevcom_gep_(C.byref(N), _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(b),
+                          _p(jb, C.c_int), _p(ib, C.c_int), _p(eigs), _p(res), _p(x),
+                          C.byref(LX), C.byref(NE), C.byref(SG), C.byref(IS), C.byref(NI),
+                          C.byref(MS), C.byref(IT), C.byref(TL), C.byref(SH), C.byref(DT),
+                          C.byref(ME), _p(ic, C.c_int), C.byref(IJ), C.byref(N1), C.byref(N2),
+                          C.byref(N3), C.byref(IP), C.byref(INFO), C.byref(GAP))
+        j = IJ.value
+        if j not in (1, 3, 4):
+            break
+        counts[j] += 1
+        xi = x[N1.value - 1: N1.value - 1 + n].copy()
+        if j in (1, 3):
+            y = np.empty(n)
+            matvec_a(xi, y)
+            x[N2.value - 1: N2.value - 1 + n] = y
+        if j in (1, 4):
+            y = np.empty(n)
+            matvec_b(xi, y)
+            x[N3.value - 1: N3.value - 1 + n] = y
+    k = NE.value if INFO.value >= 0 else 0
+    r = JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(),
+                 x=x[: n * k].reshape(k, n).T.copy(), neig=k, iter=IT.value,
+                 info=INFO.value, gap=GAP.value, shift=SH.value, droptol=DT.value)
+    r.ijob_counts = counts
+    return r
 
 
 def pjd_batch(problems, *, device=0, workers=8, want_x=False):

@@ -1,0 +1,193 @@
+"""The generalized problem A x = lambda B x on the GPU (DPJD_GEP,
+DPJDREVCOM_GEP, DJDREVCOM_GEP; src/jd/PJD.F:492-798 and the GEP branches of
+JDRVCOM, JDaux.F:1512-2372).
+
+The user guide prints no run of these drivers, so the checks are what the
+mathematics fixes:
+* eigenvalues against the dense generalized spectrum (scipy.linalg.eigh(A, B))
+  to 1e-10 relative;
+* residuals ||A x - lambda B x|| <= TOL ||B x|| recomputed on the host, and
+  B-orthonormal eigenvectors (X^T B X = I);
+* B = I reproduces the standard driver: the same eigenvalues and the same
+  MATVEC count within 3 (the generalized branches normalise in the B inner
+  product, which for B = I differs from the standard path only in rounding);
+* B = c I divides the spectrum by c;
+* the matrices of samples/EXAMPLE2_GEP.f (ILU, ISEARCH = 0) and
+  samples/EXAMPLE9_GEP.f (diagonal preconditioner, reverse communication);
+* ISEARCH = 2 is rejected with INFO = -8 and an indefinite B ends the run
+  with INFO = -54322 (JDaux.F:2146-2150).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from jadamilu_amd import matrices as M
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+N = 1000
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _fresh():
+    from jadamilu_amd import jd
+    jd.reset_legacy()
+    jd.legacy_set_option("ildl_sweeps", 0)
+    jd.legacy_set_option("cost_clock", 1)
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _restore_legacy_options():
+    yield
+    if torch.cuda.is_available():
+        from jadamilu_amd import jd
+        jd.legacy_set_option("ildl_sweeps", -1)
+        jd.legacy_set_option("cost_clock", 0)
+
+
+def _sample_pencil(diag_a):
+    """tridiag(5, diag_a, 5) and tridiag(0.25, 1, 0.25), the pencil of the
+    GEP samples (EXAMPLE2_GEP.f: diag i-1; EXAMPLE9_GEP.f: diag i)."""
+    ia, ja, a = M.tridiag(N, diag_a, 5.0)
+    ib, jb, b = M.tridiag(N, np.ones(N), 0.25)
+    return (ia, ja, a), (ib, jb, b)
+
+
+def _check(r, A, B, w_ref, neig, tol=1e-10):
+    print("matvecs", r.iter, "info", r.info, "eigs", r.eigs)
+    assert r.info == 0 and r.neig == neig, (r.info, r.neig)
+    np.testing.assert_allclose(np.sort(r.eigs), w_ref, rtol=1e-10, atol=1e-12)
+    X = r.x
+    for k in range(neig):
+        x = X[:, k]
+        bx = B @ x
+        assert np.linalg.norm(A @ x - r.eigs[k] * bx) <= 2 * tol * np.linalg.norm(bx)
+    G = X.T @ B @ X
+    np.testing.assert_allclose(G, np.eye(neig), atol=1e-8)
+
+
+def test_dpjd_gep_example2_ilu():
+    """samples/EXAMPLE2_GEP.f: the 5 smallest eigenvalues, ILU of A - shift B."""
+    from jadamilu_amd import jd
+    _dev()
+    (ia, ja, a), (ib, jb, b) = _sample_pencil(np.arange(N) * 1.0)
+    A, B = M.to_dense(ia, ja, a), M.to_dense(ib, jb, b)
+    w = sla.eigh(A, B, eigvals_only=True)
+    _fresh()
+    r = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=5, isearch=0, madspace=20, maxiter=1000,
+                        tol=1e-10, droptol=1e-3, mem=20.0)
+    st = jd.legacy_stats()
+    print("set-ups", st["n_precond_setups"], st["setup_shifts"], st["setup_fills"])
+    _check(r, A, B, w[:5], 5)
+    assert st["n_precond_setups"] >= 1
+
+
+def test_dpjdrevcom_gep_example9_diag():
+    """samples/EXAMPLE9_GEP.f: diagonal preconditioning (A = D, JA = -1),
+    A x and B x through reverse communication (IJOB 1, 3 and 4)."""
+    from jadamilu_amd import jd
+    _dev()
+    (ia, ja, a), (ib, jb, b) = _sample_pencil(np.arange(1, N + 1) * 1.0)
+    A, B = M.to_dense(ia, ja, a), M.to_dense(ib, jb, b)
+    w = sla.eigh(A, B, eigvals_only=True)
+    mva, mvb = M.to_scipy(ia, ja, a), M.to_scipy(ib, jb, b)
+    d = np.diag(A).copy()
+    db = np.diag(B).copy()
+    _fresh()
+    r = jd.pjdrevcom_gep_host(N, d, [-1], [-1], db, [-1], [-1],
+                              lambda x, y: y.__setitem__(slice(None), mva @ x),
+                              lambda x, y: y.__setitem__(slice(None), mvb @ x),
+                              neig=5, isearch=0, madspace=20, maxiter=1000, tol=1e-10)
+    print("IJOB counts", r.ijob_counts)
+    _check(r, A, B, w[:5], 5)
+    assert r.ijob_counts[1] > 0 and r.ijob_counts[3] > 0  # final checks use A only
+    assert r.ijob_counts[4] > 0  # B u of each converged pair
+
+
+@pytest.mark.parametrize("precond", ["ilu", "diag"])
+def test_identity_mass_matrix_is_the_standard_problem(precond):
+    from jadamilu_amd import jd
+    _dev()
+    ia, ja, a = M.tridiag(N, np.arange(N) * 1.0, 5.0)
+    ib, jb, b = M.tridiag(N, np.ones(N), 0.0)
+    A = M.to_dense(ia, ja, a)
+    w = np.linalg.eigvalsh(A)
+    if precond == "ilu":
+        _fresh()
+        rs = jd.pjd_host(N, a, ja, ia, neig=5, isearch=0, madspace=20, maxiter=1000, tol=1e-10,
+                         droptol=1e-3, mem=20.0)
+        _fresh()
+        rg = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=5, isearch=0, madspace=20,
+                             maxiter=1000, tol=1e-10, droptol=1e-3, mem=20.0,
+                             lx=jd.lx_optimal_gep(N, 5, 20))
+    else:
+        mva = M.to_scipy(ia, ja, a)
+        d = np.diag(A).copy()
+        _fresh()
+        rs = jd.pjdrevcom_host(N, d, [-1], [-1],
+                               lambda x, y: y.__setitem__(slice(None), mva @ x),
+                               neig=5, isearch=0, madspace=20, maxiter=1000, tol=1e-10)
+        _fresh()
+        rg = jd.pjdrevcom_gep_host(N, d, [-1], [-1], np.ones(N), [-1], [-1],
+                                   lambda x, y: y.__setitem__(slice(None), mva @ x),
+                                   lambda x, y: y.__setitem__(slice(None), x),
+                                   neig=5, isearch=0, madspace=20, maxiter=1000, tol=1e-10)
+    print(precond, "standard", rs.iter, "generalized", rg.iter)
+    assert rs.info == 0 and rg.info == 0
+    np.testing.assert_allclose(np.sort(rg.eigs), w[:5], rtol=1e-10)
+    np.testing.assert_allclose(np.sort(rg.eigs), np.sort(rs.eigs), rtol=1e-12, atol=1e-12)
+    assert abs(rg.iter - rs.iter) <= 3, (rg.iter, rs.iter)
+
+
+def test_scaled_mass_matrix():
+    from jadamilu_amd import jd
+    _dev()
+    ia, ja, a = M.tridiag(N, np.arange(N) * 1.0, 5.0)
+    ib, jb, b = M.tridiag(N, np.full(N, 4.0), 0.0)
+    w = np.linalg.eigvalsh(M.to_dense(ia, ja, a)) / 4.0
+    _fresh()
+    r = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=5, isearch=0, madspace=20, maxiter=1000,
+                        tol=1e-10, droptol=1e-3, mem=20.0)
+    assert r.info == 0
+    np.testing.assert_allclose(np.sort(r.eigs), w[:5], rtol=1e-10)
+    # B-orthonormal: ||x|| = 1/2
+    np.testing.assert_allclose(np.linalg.norm(r.x, axis=0), 0.5, rtol=1e-8)
+
+
+def test_isearch1_closest_to_shift():
+    """ISEARCH = 1 with the standard extraction: eigenvalues nearest above
+    SHIFT, the factor of A - SHIFT B."""
+    from jadamilu_amd import jd
+    _dev()
+    (ia, ja, a), (ib, jb, b) = _sample_pencil(np.arange(N) * 1.0)
+    A, B = M.to_dense(ia, ja, a), M.to_dense(ib, jb, b)
+    w = sla.eigh(A, B, eigvals_only=True)
+    _fresh()
+    r = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=4, isearch=1, madspace=20, maxiter=2000,
+                        tol=1e-10, shift=-20.0, droptol=1e-3, mem=20.0)
+    _check(r, A, B, w[:4], 4)
+
+
+def test_gep_rejections():
+    from jadamilu_amd import jd
+    _dev()
+    (ia, ja, a), (ib, jb, b) = _sample_pencil(np.arange(N) * 1.0)
+    _fresh()
+    r = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=5, isearch=2, sigma=0.0, madspace=20,
+                        maxiter=1000, tol=1e-10)
+    assert r.info == -8
+    # B = -I is negative definite: the first B-normalisation fails (diagonal
+    # preconditioner, so that no factor of A - shift B is attempted first)
+    mva = M.to_scipy(ia, ja, a)
+    _fresh()
+    r = jd.pjdrevcom_gep_host(N, np.diag(M.to_dense(ia, ja, a)).copy(), [-1], [-1],
+                              -np.ones(N), [-1], [-1],
+                              lambda x, y: y.__setitem__(slice(None), mva @ x),
+                              lambda x, y: y.__setitem__(slice(None), -x),
+                              neig=5, isearch=0, madspace=20, maxiter=1000, tol=1e-10)
+    assert r.info == -54322
