@@ -15,7 +15,7 @@
  *   dpjdrevcom_gep_ DPJDREVCOM_GEP src/jd/PJD.F:576-703   (jadamilu1.h:44-53)
  *   djdrevcom_gep_  DJDREVCOM_GEP  src/jd/PJD.F:709-798   (jadamilu1.h:54-59)
  * The generalized drivers solve A x = lambda B x with B symmetric positive
- * definite, for ISEARCH <= 1 (ISEARCH = 2 returns INFO = -8); their reverse
+ * definite (all ISEARCH modes); their reverse
  * communication adds NDX3 and the codes IJOB = 1 (X(NDX2) = A X(NDX1) and
  * X(NDX3) = B X(NDX1)), 3 (A only) and 4 (X(NDX3) = B X(NDX1) only); a mass
  * matrix found not positive definite ends the run with INFO = -54322.
@@ -148,6 +148,14 @@ void jdhip_jdrevcom_gep(jdhip_ctx *ctx, int n, double *eigs, double *res, double
                         int iprint, int *info, double *gap);
 /* As DPJDCLEANUP (PJD.F:435): release the preconditioner of ctx. */
 void jdhip_cleanup(jdhip_ctx *ctx);
+
+/* Eigenvalues wr + i wi and right eigenvectors v (column-major, ld ldv) of a
+ * small real general n x n matrix a (overwritten): the DGEEV step of
+ * SOLPROJEP for the harmonic extraction of the generalized problem
+ * (JDaux.F:3293-3300), exposed for checking.  A complex pair stores the real
+ * and imaginary parts of the first eigenvalue's vector in two columns.
+ * 0 on success, k > 0 if the QR iteration did not converge, -1 bad input. */
+int jdhip_dense_eig(int n, double *a, int lda, double *wr, double *wi, double *v, int ldv);
 
 /* Approximate minimum degree ordering of the symmetric pattern given by a
  * 1-based upper (or lower) triangle CSR; perm[k] (1-based) is the original

@@ -85,6 +85,7 @@ SIGNATURES = {
                                   C.c_int, C.c_int, _i, C.c_double, _i, _l, _l, _l, C.c_int,
                                   _i, _d]),
     "jdhip_cleanup": (None, [_v]),
+    "jdhip_dense_eig": (C.c_int, [C.c_int, _d, C.c_int, _d, _d, _d, C.c_int]),
     "jdhip_reset": (C.c_int, [_v]),
     "jdhip_order_amd": (C.c_int, [C.c_int, _i, _i, _i]),
     "jdhip_legacy_context": (_v, []),

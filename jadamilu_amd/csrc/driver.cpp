@@ -23,6 +23,7 @@
 #include "ordering.hpp"
 #include "../../include/jadamilu_amd.h"
 #include "engine.hpp"
+#include "dense.hpp"
 #include "ildl.hpp"
 #include "kernels.hpp"
 
@@ -1774,6 +1775,11 @@ void djdrevcom_gep(int *n, double *eigs, double *res, double *x, int *lx, int *n
                    int *info, double *gap) {
     djdrevcom_gep_(n, eigs, res, x, lx, neig, sigma, isearch, ninit, madspace, iter, tol, ijob,
                    ndx1, ndx2, ndx3, iprint, info, gap);
+}
+
+int jdhip_dense_eig(int n, double *a, int lda, double *wr, double *wi, double *v, int ldv) {
+    if (n < 0 || lda < n || ldv < n || (n > 0 && (!a || !wr || !wi || !v))) return -1;
+    return jdhip::real_eig(n, a, lda, wr, wi, v, ldv);
 }
 
 void dpjdcleanup_(void) {

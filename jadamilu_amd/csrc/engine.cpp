@@ -240,13 +240,26 @@ void JdEngine::sortev_dev(int n, int j, long voff, double *eigs, double *r, int 
     }
 }
 
-// SOLPROJEP (JDaux.F:3271-3358), standard problem, on host matrices.
+// SOLPROJEP (JDaux.F:3271-3358) on host matrices: SYEV (|is| != 2), SYGV
+// (harmonic, standard problem) or POSV + GEEV (harmonic, generalized).
 int JdEngine::solprojep(int j, double *mm, double *nn, double *th, double *th2,
                         std::vector<double> &w, int is) {
     if ((int)w.size() < std::max(j, 2)) w.resize(std::max(j, 2), 0.0);
     int info;
     if (is == 2 || is == -2) {
-        info = sym_gen_eig(j, mm, modspace, nn, modspace, w.data());
+        if (gep_) {
+            // JDaux.F:3293-3300: MM := NN^-1 MM (POSV), its eigenvectors
+            // (GEEV, the real parts of the eigenvalues kept) over MM
+            info = chol_solve(j, nn, modspace, j, mm, modspace);
+            for (int c = 0; c < j; ++c)
+                std::memcpy(nn + (long)c * modspace, mm + (long)c * modspace,
+                            sizeof(double) * j);
+            if (info) return info;
+            std::vector<double> wi(j);
+            info = real_eig(j, nn, modspace, w.data(), wi.data(), mm, modspace);
+        } else {
+            info = sym_gen_eig(j, mm, modspace, nn, modspace, w.data());
+        }
         if (info) return info;
         // SORTEV on the small eigenvector matrix (JDaux.F:3350-3352)
         if (j > 1) {
@@ -308,8 +321,9 @@ int JdEngine::shrink(int n, int j, int k, int ucol_off, int iserch) {
                                    hipMemcpyDeviceToDevice, s));
     }
     const int ld = modspace;
-    for (int a = 1; a < k; ++a)
-        for (int b = 0; b < a; ++b) MM[a + b * ld] = MM[b + a * ld];
+    if (iserch != 2 || !gep_)  // MM is full (not symmetric) for the harmonic GEP
+        for (int a = 1; a < k; ++a)
+            for (int b = 0; b < a; ++b) MM[a + b * ld] = MM[b + a * ld];
     apply_q_right(k, k, j, U, ld, tau.data(), MM.data(), ld);
     apply_qt_left(k, k, j, U, ld, tau.data(), MM.data(), ld);
     if (iserch == 2) {
@@ -335,6 +349,14 @@ void JdEngine::gram_initial(int n) {
         fetch(C.data(), ctx_->d_gout, (long)J * J);
         for (int c = 0; c < J; ++c)
             for (int r = 0; r <= c; ++r) NN[r + c * ld] = C[r + (size_t)c * J];
+    }
+    if (iserch == 2 && gep_) {
+        // MM = W^T (B V), not symmetric (JDaux.F:1653-1655)
+        gram(s, n, J, J, X_ + ww, n, X_ + bv, n, ctx_->d_gout, J, ctx_->d_gram);
+        fetch(C.data(), ctx_->d_gout, (long)J * J);
+        for (int c = 0; c < J; ++c)
+            for (int r = 0; r < J; ++r) MM[r + c * ld] = C[r + (size_t)c * J];
+        return;
     }
     // MM = (V^T W + W^T V) / 2 from one Gram product C = V^T W
     gram_sym(n, X_ + vv, X_ + ww, J, C.data());
@@ -549,8 +571,10 @@ L1:
     else if (gep_ && (long)n * (3 * madspace + 2 * *neig + 4) + 3L * madspace * madspace +
                          std::max(madspace * madspace, *neig) > lx)
         *info = -10;  // JDaux.F:1512-1515
-    else if (gep_ && isearch >= 2)
-        *info = -8;  // harmonic extraction of the generalized problem: not supported
+    else if (gep_ && isearch >= 2 &&
+             (long)n * (3 * madspace + 2 * *neig) + 6L * madspace + 3L * madspace * madspace +
+                     std::max(madspace * madspace, *neig) > lx)
+        *info = -10;  // JDaux.F:1516-1519
     else if (*iter <= 0) *info = -11;
     else if (tol <= 0.0) *info = -12;
     else if (std::min(*neig, n) + std::min(madspace, n) > kMaxCols)
@@ -591,6 +615,7 @@ L1:
     NP.assign((size_t)modspace * modspace, 0.0);
     WL.assign(3 * modspace + 4, 0.0);
     WL2.assign(modspace + 2, 0.0);
+    WL3.assign(modspace + 2, 0.0);
     REIGW.assign(std::max(modspace, 2) + 2, 0.0);
     ycoef.assign(2 * (nwant + modspace) + 4, 0.0);
     epsmsr = std::sqrt(kEps);
@@ -696,8 +721,13 @@ L10:  // extraction
     }
     if (gep_) nbu = dnorm(D(bu), n);
     if (iserch == 2) {
-        symv_upper(J, MM.data(), modspace, YY(), WL.data());
-        thmt = small_dot(J, WL.data(), YY());
+        if (gep_) {  // MM is not symmetric here (JDaux.F:1699-1702)
+            gemv_host(J, J, MM.data(), modspace, YY(), WL.data(), false);
+            thmt = small_dot(J, WL.data(), YY()) / (nbu * nbu);
+        } else {
+            symv_upper(J, MM.data(), modspace, YY(), WL.data());
+            thmt = small_dot(J, WL.data(), YY());
+        }
         theta = thmt + sigma;
     } else {
         thmt = theta;
@@ -820,10 +850,14 @@ L20:  // expansion
     if (J > 1) {
         deltao = delta1;
         if (iserch == 2) {
-            sclr1 = small_nrm2(J, YY2());
-            for (int c = 0; c < J; ++c) YY2()[c] *= 1.0 / sclr1;
-            symv_upper(J, MM.data(), modspace, YY2(), WL.data());
-            theta2 = small_dot(J, YY2(), WL.data());
+            if (gep_) {
+                theta2 = 1.0 / theta2;
+            } else {
+                sclr1 = small_nrm2(J, YY2());
+                for (int c = 0; c < J; ++c) YY2()[c] *= 1.0 / sclr1;
+                symv_upper(J, MM.data(), modspace, YY2(), WL.data());
+                theta2 = small_dot(J, YY2(), WL.data());
+            }
             delta1 = std::fabs(theta2) - std::fabs(theta - sigma);
         } else {
             delta1 = theta2 - theta;
@@ -1160,7 +1194,17 @@ L55:
     YY()[J] = -sclr1;
     axpyalpha = -sigma;
     if (iserch == 2) axpy(s, D(*ndx2), axpyalpha, D(bv + (long)J * n), n);
-    {
+    if (iserch == 2 && gep_) {
+        // MM(0:J, J) = W^T (B v), MM(J, 0:J-1) = (B V)^T w (JDaux.F:2012-2016)
+        double buf[3 * kMaxCols];
+        multidot(s, rb, D(ww), n, J + 1, D(bv + (long)J * n), n, ctx_->slot(0));
+        multidot(s, rb, D(ww), n, J + 1, D(*ndx2), n, ctx_->slot(1));
+        if (J > 0) multidot(s, rb, D(bv), n, J, D(*ndx2), n, ctx_->slot(2));
+        fetch(buf, ctx_->slot(0), 3 * kMaxCols);
+        for (int r = 0; r <= J; ++r) MM[r + (long)J * modspace] = buf[r];
+        for (int c = 0; c < J; ++c) MM[J + (long)c * modspace] = buf[2 * kMaxCols + c];
+        for (int r = 0; r <= J; ++r) NN[r + (long)J * modspace] = buf[kMaxCols + r];
+    } else {
         double buf[2 * kMaxCols];
         multidot(s, rb, D(vv), n, J + 1, D(*ndx2), n, ctx_->slot(0));
         if (iserch == 2) multidot(s, rb, D(ww), n, J + 1, D(*ndx2), n, ctx_->slot(1));
@@ -1173,9 +1217,25 @@ L55:
     uknown = false;
     if (!conv) goto L10;
     // convergence detected: prepare for restart (JDaux.F:2040-2081)
-    sclr1 = small_nrm2(J, YY());
-    for (int c = 0; c < J; ++c) YY()[c] *= 1.0 / sclr1;
-    if (iserch == 2) {
+    if (iserch != 2 || !gep_) {
+        sclr1 = small_nrm2(J, YY());
+        for (int c = 0; c < J; ++c) YY()[c] *= 1.0 / sclr1;
+    }
+    if (iserch == 2 && gep_) {
+        symv_upper(J, NN.data(), modspace, YY(), WL2.data());
+        sclr1 = small_dot(J, YY(), WL2.data());
+        gemv_host(J, J, MM.data(), modspace, YY(), WL.data(), false);
+        axpyalpha = small_dot(J, YY(), WL.data());
+        thmt = axpyalpha / (alprho * nbu * nbu);
+        axpyalpha = -axpyalpha / sclr1;
+        for (int c = 0; c < J; ++c) WL[c] += axpyalpha * WL2[c];
+        gemv_host(J, J, MM.data(), modspace, YY(), WL3.data(), true);
+        MP = MM;
+        ger_host(J, J, -1.0 / sclr1, WL.data(), WL2.data(), MP.data(), modspace);
+        ger_host(J, J, -1.0 / sclr1, WL2.data(), WL3.data(), MP.data(), modspace);
+        theta = thmt + sigma;
+        NP = NN;
+    } else if (iserch == 2) {
         symv_upper(J, NN.data(), modspace, YY(), WL2.data());
         sclr1 = small_dot(J, YY(), WL2.data());
         symv_upper(J, MM.data(), modspace, YY(), WL.data());

@@ -88,7 +88,7 @@ class JdEngine {
     // X(ndx1)=P^-1 X(ndx2), 3 matvec X(ndx2)=A X(ndx1) only (final checks),
     // 4 X(ndx3)=B X(ndx1) only.  ndx are 0-based offsets into x.  gep (read
     // when *ijob = 0): A x = lambda B x with B symmetric positive definite
-    // (JDRVCOM's GEP branches), ISEARCH <= 1.
+    // (JDRVCOM's GEP branches).
     void step(int n, double *eigs, double *res, double *x, long lx, int *neig, double sigma,
               int isearch, int ninit, int madspace, int *iter, double tol, int *ijob,
               long *ndx1, long *ndx2, long *ndx3, int iprint, int *info, double *gap,

@@ -486,6 +486,29 @@ class Context:
                         iter=IT.value, info=INFO.value, gap=GAP.value, shift=SH.value,
                         droptol=DT.value, stats=self.stats())
 
+    def pjd_gep(self, n, a, ja, ia, b, jb, ib, xdev_ptr, lx, *, neig, isearch=0, sigma=0.0,
+                ninit=0, madspace=20, maxiter=1000, tol=1e-10, shift=0.0, droptol=1e-3,
+                mem=20.0, icntl=(0, 0, 0, 0, 0), eigs0=None, iprint=0):
+        """jdhip_pjd_gep: A x = lambda B x with device products by A and B."""
+        L = self._L
+        a, b = np.array(a, dtype=np.float64), np.array(b, dtype=np.float64)
+        ja, jb = np.array(ja, dtype=np.int32), np.array(jb, dtype=np.int32)
+        ia, ib = np.array(ia, dtype=np.int32), np.array(ib, dtype=np.int32)
+        eigs = np.zeros(max(neig, 1)) if eigs0 is None else np.array(eigs0, dtype=np.float64)
+        res = np.zeros(max(neig, 1))
+        ic = _icntl(icntl)
+        NE, IT, SH, DT, ME = _i(neig), _i(maxiter), _d(shift), _d(droptol), _d(mem)
+        INFO, GAP = _i(0), _d(0)
+        L.jdhip_pjd_gep(self.h, n, _p(a), _p(ja, C.c_int), _p(ia, C.c_int), _p(b),
+                        _p(jb, C.c_int), _p(ib, C.c_int), _p(eigs), _p(res),
+                        C.c_void_p(xdev_ptr), lx, C.byref(NE), sigma, isearch, ninit, madspace,
+                        C.byref(IT), tol, C.byref(SH), C.byref(DT), C.byref(ME),
+                        _p(ic, C.c_int), iprint, C.byref(INFO), C.byref(GAP))
+        k = NE.value if INFO.value >= 0 else 0
+        return JDResult(eigs=eigs[:k].copy(), res=res[:k].copy(), x=None, neig=k,
+                        iter=IT.value, info=INFO.value, gap=GAP.value, shift=SH.value,
+                        droptol=DT.value, stats=self.stats())
+
     def jdrevcom(self, n, xdev_ptr, lx, *, neig, matvec, psolve, isearch=0, sigma=0.0, ninit=0,
                  madspace=20, maxiter=1000, tol=1e-10, eigs0=None, iprint=0):
         """jdhip_jdrevcom: matvec(xin_ptr, xout_ptr), psolve(src_ptr, dst_ptr)."""

@@ -14,8 +14,9 @@ mathematics fixes:
 * B = c I divides the spectrum by c;
 * the matrices of samples/EXAMPLE2_GEP.f (ILU, ISEARCH = 0) and
   samples/EXAMPLE9_GEP.f (diagonal preconditioner, reverse communication);
-* ISEARCH = 2 is rejected with INFO = -8 and an indefinite B ends the run
-  with INFO = -54322 (JDaux.F:2146-2150).
+* the harmonic extraction (ISEARCH = 2, samples/EXAMPLE1_GEP.f: eigenvalues
+  nearest SIGMA) against the dense spectrum, on the device path as well;
+* an indefinite B ends the run with INFO = -54322 (JDaux.F:2146-2150).
 """
 import numpy as np
 import pytest
@@ -58,15 +59,21 @@ def _sample_pencil(diag_a):
     return (ia, ja, a), (ib, jb, b)
 
 
-def _check(r, A, B, w_ref, neig, tol=1e-10):
-    print("matvecs", r.iter, "info", r.info, "eigs", r.eigs)
+def _check(r, A, B, w_ref, neig, tol=1e-10, slack=2.0):
+    """slack: bound of RES and of the host-recomputed residual in units of
+    TOL ||B x||.  RES is the true residual recomputed at termination; a pair
+    whose convergence was declared inside the inner solve, on STOPTEST's
+    estimate, can end a few times above TOL, as in the reference (observed:
+    2.7 TOL for one pair of the harmonic run with SIGMA = 3)."""
+    print("matvecs", r.iter, "info", r.info, "eigs", r.eigs, "res", r.res)
     assert r.info == 0 and r.neig == neig, (r.info, r.neig)
     np.testing.assert_allclose(np.sort(r.eigs), w_ref, rtol=1e-10, atol=1e-12)
+    assert (r.res <= slack * tol).all(), r.res
     X = r.x
     for k in range(neig):
         x = X[:, k]
         bx = B @ x
-        assert np.linalg.norm(A @ x - r.eigs[k] * bx) <= 2 * tol * np.linalg.norm(bx)
+        assert np.linalg.norm(A @ x - r.eigs[k] * bx) <= slack * tol * np.linalg.norm(bx)
     G = X.T @ B @ X
     np.testing.assert_allclose(G, np.eye(neig), atol=1e-8)
 
@@ -173,14 +180,27 @@ def test_isearch1_closest_to_shift():
     _check(r, A, B, w[:4], 4)
 
 
+@pytest.mark.parametrize("sigma", [0.0, 3.0])
+def test_dpjd_gep_example1_harmonic(sigma):
+    """samples/EXAMPLE1_GEP.f: the 5 eigenvalues nearest SIGMA (ISEARCH = 2,
+    harmonic Ritz values from POSV + GEEV of the projected pencil)."""
+    from jadamilu_amd import jd
+    _dev()
+    (ia, ja, a), (ib, jb, b) = _sample_pencil(np.arange(N) * 1.0)
+    A, B = M.to_dense(ia, ja, a), M.to_dense(ib, jb, b)
+    w = sla.eigh(A, B, eigvals_only=True)
+    near = np.sort(w[np.argsort(np.abs(w - sigma))[:5]])
+    _fresh()
+    r = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=5, isearch=2, sigma=sigma, madspace=20,
+                        maxiter=1000, tol=1e-10, droptol=1e-3, mem=20.0)
+    print("sigma", sigma, "gap", r.gap)
+    _check(r, A, B, near, 5, slack=5.0)
+
+
 def test_gep_rejections():
     from jadamilu_amd import jd
     _dev()
     (ia, ja, a), (ib, jb, b) = _sample_pencil(np.arange(N) * 1.0)
-    _fresh()
-    r = jd.pjd_gep_host(N, a, ja, ia, b, jb, ib, neig=5, isearch=2, sigma=0.0, madspace=20,
-                        maxiter=1000, tol=1e-10)
-    assert r.info == -8
     # B = -I is negative definite: the first B-normalisation fails (diagonal
     # preconditioner, so that no factor of A - shift B is attempted first)
     mva = M.to_scipy(ia, ja, a)
@@ -191,3 +211,40 @@ def test_gep_rejections():
                               lambda x, y: y.__setitem__(slice(None), -x),
                               neig=5, isearch=0, madspace=20, maxiter=1000, tol=1e-10)
     assert r.info == -54322
+
+
+def test_device_path_3d_pencil():
+    """jdhip_pjd_gep on a 3D pencil (7-point Laplacian on a 20 x 23 x 26 grid
+    with the mass matrix of diagonal 1 and neighbour weight 1/12, SPD by
+    diagonal dominance), X in device memory, tuned sweeps: the 6 smallest
+    eigenvalues and the 4 nearest 0.09 (an interior target) against a
+    shift-invert Lanczos reference."""
+    import scipy.sparse.linalg as spl
+    from jadamilu_amd.jd import Context, lx_optimal_gep
+    _dev()
+    shape = (20, 23, 26)
+    n = int(np.prod(shape))
+    ia, ja, a = M.laplace(shape)
+    ib, jb, b = M.stencil(shape, 1.0, 1.0 / 12.0)
+    A, B = M.to_scipy(ia, ja, a).tocsc(), M.to_scipy(ib, jb, b).tocsc()
+    for neig, isearch, sigma in [(6, 0, 0.0), (4, 2, 0.09)]:
+        w = spl.eigsh(A, k=neig + 4, M=B, sigma=sigma, which="LM", return_eigenvectors=False,
+                      tol=1e-13)
+        ref = np.sort(w[np.argsort(np.abs(w - sigma))[:neig]])
+        ctx = Context(0)
+        lx = lx_optimal_gep(n, neig, 20)
+        X = torch.zeros(lx, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        r = ctx.pjd_gep(n, a, ja, ia, b, jb, ib, X.data_ptr(), lx, neig=neig, isearch=isearch,
+                        sigma=sigma, madspace=20, maxiter=4000, tol=1e-10, droptol=1e-3,
+                        mem=20.0, icntl=(0, 1, 0, 0, 0))
+        torch.cuda.synchronize()
+        xs = X[: n * r.neig].reshape(r.neig, n).T.cpu().numpy()
+        ctx.close()
+        print("3d pencil isearch", isearch, "matvecs", r.iter, "info", r.info)
+        assert r.info == 0 and r.neig == neig
+        np.testing.assert_allclose(np.sort(r.eigs), ref, rtol=1e-10)
+        for k in range(neig):
+            x = xs[:, k]
+            bx = B @ x
+            assert np.linalg.norm(A @ x - r.eigs[k] * bx) <= 2e-10 * np.linalg.norm(bx)
