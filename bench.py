@@ -43,6 +43,10 @@ WORKLOADS = {
     "laplace3d_128": ("laplace", (128, 128, 128), 10, 20, 1, 0.0, 6.0),
     "laplace2d_100": ("laplace", (100, 100), 10, 20, 1, 0.0, 6.0),
     "laplace3d_64": ("laplace", (64, 64, 64), 10, 20, 1, 0.0, 6.0),
+    # generalized problem A x = lambda B x (DPJD_GEP): A the 7-point
+    # Laplacian 128^3, B the 7-point mass-like stencil with diagonal 1 and
+    # neighbour weight 1/12 (B = 1.5 I - A / 12, SPD), C2's solver settings
+    "laplace3d_128_gep": ("laplace_gep", (128, 128, 128), 10, 20, 1, 0.0, 6.0),
     "anderson_100": ("anderson", (100, 16.5, 0), 10, 25, 2, 0.0, 25.0),
     "anderson_40": ("anderson", (40, 16.5, 0), 4, 25, 2, 0.0, 25.0),
     # 256 independent Anderson problems, L uniform in {16..48}, W uniform in
@@ -60,7 +64,7 @@ WORKLOADS = {
 
 def build_matrix(kind, params):
     from jadamilu_amd import matrices as M
-    if kind == "laplace":
+    if kind in ("laplace", "laplace_gep"):
         return M.laplace(params)
     L, W, seed = params
     return M.anderson(L, W, seed)
@@ -144,6 +148,37 @@ def kernel_probe(ctx, n, nnz_u, reps, dev, torch, sweep=None):
     out.update({"spmv_avg_ms": spmv_ms, "spmv_alg_bytes": spmv_bytes(n, nnz_u),
                 "copy_avg_ms": copy_ms, "copy_bytes_each_way": 1 << 28,
                 "copy_GBps_read_plus_write": 2 * (1 << 28) / (copy_ms * 1e-3) / 1e9})
+    # the dense contractions of the outer iteration on the FP64 matrix cores,
+    # at the shapes of a MADSPACE = 20 run: the Gram product V^T W (20 x 20,
+    # the Rayleigh-Ritz blocks) and the basis update V Q (20 -> 10, SHRINK)
+    K = 20
+    V = torch.rand(n * K, dtype=torch.float64, device=dev)
+    W = torch.rand(n * K, dtype=torch.float64, device=dev)
+    Cm = torch.empty(K * K, dtype=torch.float64, device=dev)
+    Q = torch.rand(K * 10, dtype=torch.float64, device=dev)
+    Y = torch.empty(n * 10, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
+    g0.record(stream)
+    for _ in range(reps):
+        ctx.gram(n, K, K, V.data_ptr(), n, W.data_ptr(), n, Cm.data_ptr(), K)
+    g1.record(stream)
+    g1.synchronize()
+    gram_ms = g0.elapsed_time(g1) / reps
+    g0.record(stream)
+    for _ in range(reps):
+        ctx.tall_gemm(n, K, 10, V.data_ptr(), n, Q.data_ptr(), K, Y.data_ptr(), n)
+    g1.record(stream)
+    g1.synchronize()
+    tg_ms = g0.elapsed_time(g1) / reps
+    gram_bytes = 8 * n * 2 * K
+    tg_bytes = 8 * n * (K + 10)
+    out.update({"gram_20x20_avg_ms": gram_ms, "gram_alg_bytes": gram_bytes,
+                "gram_GBps": gram_bytes / (gram_ms * 1e-3) / 1e9,
+                "gram_TFLOPs": 2.0 * n * K * K / (gram_ms * 1e-3) / 1e12,
+                "tall_gemm_20to10_avg_ms": tg_ms, "tall_gemm_alg_bytes": tg_bytes,
+                "tall_gemm_GBps": tg_bytes / (tg_ms * 1e-3) / 1e9})
     print(json.dumps(out))
 
 
@@ -437,7 +472,14 @@ def main():
     t0 = time.time()
     ctx.set_matrix(ia, ja, a)  # upload + SpMV lanes autotune (outside the timed region)
     setup_s = time.time() - t0
-    lx = lx_optimal(n, neig, madspace)
+    gep = kind == "laplace_gep"
+    if gep:
+        from jadamilu_amd import matrices as M
+        from jadamilu_amd.jd import lx_optimal_gep
+        ib, jb, b = M.stencil(params, 1.0, 1.0 / 12.0)
+        lx = lx_optimal_gep(n, neig, madspace)
+    else:
+        lx = lx_optimal(n, neig, madspace)
     X = torch.zeros(lx, dtype=torch.float64, device=dev)
     x0_dev = torch.tensor(x0, device=dev)
     torch.cuda.synchronize()
@@ -466,6 +508,11 @@ def main():
                                  isearch=isearch, sigma=sigma, ninit=1, madspace=madspace,
                                  maxiter=args.maxiter, tol=args.tol,
                                  icntl=(0, icntl2, 0, 0, 0))
+        if gep:
+            return ctx.pjd_gep(n, a, ja, ia, b, jb, ib, X.data_ptr(), lx, neig=neig,
+                               isearch=isearch, sigma=sigma, ninit=1, madspace=madspace,
+                               maxiter=args.maxiter, tol=args.tol, droptol=args.droptol, mem=mem,
+                               icntl=(0, icntl2, 0, 0, single))
         return ctx.pjd(n, a, ja, ia, X.data_ptr(), lx, neig=neig, isearch=isearch, sigma=sigma,
                        ninit=1, madspace=madspace, maxiter=args.maxiter, tol=args.tol,
                        droptol=args.droptol, mem=mem, icntl=(0, icntl2, 0, 0, single))
@@ -507,9 +554,11 @@ def main():
 
     # accuracy against the closed form (Laplacians) -- reported, not asserted
     max_rel_err = None
-    if kind == "laplace":
+    if kind in ("laplace", "laplace_gep"):
         from jadamilu_amd.matrices import laplace_eigs
         lam, _ = laplace_eigs(params, neig)
+        if gep:  # B = 1.5 I - A / 12 shares A's eigenvectors
+            lam = lam / (1.5 - lam / 12.0)
         if r.neig == neig:
             max_rel_err = float(np.max(np.abs(np.sort(r.eigs) - lam) / np.abs(lam)))
 
@@ -600,7 +649,7 @@ def main():
 
     # ---- CPU baseline: oracle on a bounded sample of the same solve ----
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not gep:  # the oracle restates the standard problem only
         from oracle import oracle as O
         native = O.use_native()  # -O3 -march=native, built on this host
         fac_s = 0.0
@@ -668,6 +717,8 @@ def main():
                   "f32 factor + f64 inner/outer") if args.single else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz_upper": nnz_u, "neig": neig,
+                   "problem": ("generalized A x = lambda B x, B the 7-point stencil with "
+                               "diagonal 1 and neighbour weight 1/12" if gep else "standard"),
                    "tol": args.tol, "madspace": madspace, "isearch": isearch, "sigma": sigma,
                    "precond": ("diagonal (DPJDREVCOM JA(1)<0)" if args.precond == "diag" else
                                f"multilevel ILDL droptol {args.droptol} "
