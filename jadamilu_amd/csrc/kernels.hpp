@@ -61,12 +61,12 @@ void tall_gemm(hipStream_t s, const double *V, long ldv, int K, const double *Q,
 // ---- FP64 matrix-core (MFMA) contractions, kernels_mfma.hip ----
 // Partial sums of gram() hold at most this many doubles (4 MiB).
 constexpr long kGramScratch = 524288;
-// Workgroups of gram() for these sizes: at most 512, at most one per 64 rows,
+// Workgroups of gram() for these sizes: at most 1024, at most one per 64 rows,
 // and nblk * K * M <= kGramScratch.
 int gram_blocks(long n, int K, int M);
 // C(i + j*ldc) = sum_r V[i*ldv + r] W[j*ldw + r]  (C = V^T W, K, M <= 64),
 // with gram_blocks(n, K, M) * K * M doubles of partial sums in scratch
-// (device).  Deterministic (partials summed in block order).  0 on success,
+// (device).  Deterministic (the same summation order on every call).  0 on success,
 // -1 if K or M exceeds 64.
 int gram(hipStream_t s, long n, int K, int M, const double *V, long ldv, const double *W,
          long ldw, double *C, int ldc, double *scratch);

@@ -35,8 +35,12 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 }
 
 // Partial Gram of one workgroup: rows r = blockIdx.x*kGR, step gridDim.x*kGR.
-// KT, MT: number of 16-wide tiles of K and M.  partials: gridDim.x blocks of
-// K*M doubles (block-major, C(i,j) at i + j*K).
+// KT, MT: number of 16-wide tiles of K and M.  partials: K*M entries of
+// gridDim.x doubles each (entry-major, C(i,j) of block b at
+// (i + j*K) * gridDim.x + b), so that the finish reads each entry's partial
+// sums contiguously.  The next step's rows are loaded into registers while
+// the matrix cores work on the current step (one LDS buffer, register
+// double buffering).
 template <int KT, int MT>
 __global__ __launch_bounds__(kGB) void k_gram(const double *__restrict__ V, long ldv, int K,
                                               const double *__restrict__ W, long ldw, int M,
@@ -50,17 +54,34 @@ __global__ __launch_bounds__(kGB) void k_gram(const double *__restrict__ V, long
     v4d acc[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (long r0 = (long)blockIdx.x * kGR; r0 < n; r0 += (long)gridDim.x * kGR) {
-        // stage: column c, row q of the step -> s[c * kGPad + q]; four columns
-        // per pass of the 256 threads, each column's 64 rows contiguous
-        const int q = threadIdx.x & 63;
-        const long row = r0 + q;
+    // staging: thread (wave w, lane q) holds row q of the step for the
+    // columns w, w + 4, w + 8, ...; each column's 64 rows are one wave's
+    // contiguous 512-byte load
+    double rv[KT * 4], rw[MT * 4];
+    auto load = [&](long r0) {
+        const long row = r0 + lane;
         const bool in = row < n;
-        for (int c = threadIdx.x >> 6; c < KT * 16; c += kGB / 64)
-            sv[c * kGPad + q] = (in && c < K) ? V[(long)c * ldv + row] : 0.0;
-        for (int c = threadIdx.x >> 6; c < MT * 16; c += kGB / 64)
-            sw[c * kGPad + q] = (in && c < M) ? W[(long)c * ldw + row] : 0.0;
+#pragma unroll
+        for (int j = 0; j < KT * 4; ++j) {
+            const int c = wave + 4 * j;
+            rv[j] = (in && c < K) ? V[(long)c * ldv + row] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < MT * 4; ++j) {
+            const int c = wave + 4 * j;
+            rw[j] = (in && c < M) ? W[(long)c * ldw + row] : 0.0;
+        }
+    };
+    const long stride = (long)gridDim.x * kGR;
+    long r0 = (long)blockIdx.x * kGR;
+    if (r0 < n) load(r0);
+    for (; r0 < n; r0 += stride) {
+#pragma unroll
+        for (int j = 0; j < KT * 4; ++j) sv[(wave + 4 * j) * kGPad + lane] = rv[j];
+#pragma unroll
+        for (int j = 0; j < MT * 4; ++j) sw[(wave + 4 * j) * kGPad + lane] = rw[j];
         __syncthreads();
+        if (r0 + stride < n) load(r0 + stride);
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
             const int tile = wave + 4 * t;
@@ -74,7 +95,7 @@ __global__ __launch_bounds__(kGB) void k_gram(const double *__restrict__ V, long
         }
         __syncthreads();
     }
-    double *P = partials + (long)blockIdx.x * K * M;
+    const long nb = gridDim.x;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         const int tile = wave + 4 * t;
@@ -84,22 +105,27 @@ __global__ __launch_bounds__(kGB) void k_gram(const double *__restrict__ V, long
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = ti * 16 + 4 * r + lk;
-                if (i < K && j < M) P[i + (long)j * K] = acc[t][r];
+                if (i < K && j < M) partials[(i + (long)j * K) * nb + blockIdx.x] = acc[t][r];
             }
         }
     }
 }
 
-// C(i, j) = sum over the blocks of the partials, in block order; one thread
-// per output entry.
+// C(i, j) = sum over the blocks of the partials: one wave per output entry,
+// lane l adds the blocks l, l + 64, ... in order, then a fixed butterfly over
+// the 64 lanes (the same summation order on every call).
 __global__ __launch_bounds__(kGB) void k_gram_finish(const double *__restrict__ partials,
                                                      int nblk, int K, int M,
                                                      double *__restrict__ C, int ldc) {
-    const int e = blockIdx.x * kGB + threadIdx.x;
-    if (e >= K * M) return;
+    const int e = blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e >= K * M) return;  // the whole wave leaves together
+    const double *p = partials + (long)e * nblk;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partials[(long)b * K * M + e];
-    C[(e % K) + (long)(e / K) * ldc] = s;
+    for (int b = lane; b < nblk; b += 64) s += p[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) C[(e % K) + (long)(e / K) * ldc] = s;
 }
 
 // Y = V Q: one wave per 16-row tile of Y, all J columns (JT tiles); Q staged
@@ -159,7 +185,7 @@ int gram_blocks(long n, int K, int M) {
     // K x M, and no more blocks than 64-row steps
     long cap = kGramScratch / std::max(1, K * M);
     long steps = (n + kGR - 1) / kGR;
-    long b = std::min<long>(std::min<long>(512, cap), std::max<long>(steps, 1));
+    long b = std::min<long>(std::min<long>(1024, cap), std::max<long>(steps, 1));
     return (int)std::max<long>(b, 1);
 }
 
@@ -183,7 +209,8 @@ int gram(hipStream_t s, long n, int K, int M, const double *V, long ldv, const d
     else JDHIP_GRAM_M(4);
 #undef JDHIP_GRAM_M
 #undef JDHIP_GRAM
-    k_gram_finish<<<(K * M + kGB - 1) / kGB, kGB, 0, s>>>(scratch, nblk, K, M, C, ldc);
+    k_gram_finish<<<(K * M + kGB / 64 - 1) / (kGB / 64), kGB, 0, s>>>(scratch, nblk, K, M, C,
+                                                                       ldc);
     return 0;
 }
 
