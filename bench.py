@@ -59,6 +59,10 @@ WORKLOADS = {
     # eigenvalues nearest 0 of Anderson L=40, W=16.5: dV (n) and dA (CSR),
     # checked by central differences (h = 1e-6) on 64 + 16 coordinates
     "anderson_40_adjoint": ("adjoint", (40, 16.5, 0), 4, 25, 2, 0.0, 25.0),
+    # complex Hermitian (ZPJD): Anderson L=64, W=16.5, in a magnetic field of
+    # flux 1/8 per plaquette (Landau gauge), 10 eigenpairs nearest 0, MEM 15;
+    # solved as the real symmetric form of order 2 L^3
+    "magnetic_anderson_64": ("complex", (64, 16.5, 0.125, 0), 10, 25, 2, 0.0, 15.0),
 }
 
 
@@ -199,8 +203,14 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
                           madspace=madspace, maxiter=args.maxiter, tol=args.tol,
                           droptol=args.droptol, mem=mem, icntl=(0, 0, 0, 0, 1)))
     print(f"# batch of {count} problems, total n {ntot}", file=sys.stderr, flush=True)
+    bopts = {}
+    if args.graph:
+        bopts["ildl_graph"] = 1
+    for kv in args.set:
+        name, value = kv.split("=", 1)
+        bopts[name] = float(value)
     for w in range(args.warmup):
-        jd.pjd_batch(probs, workers=args.workers)
+        jd.pjd_batch(probs, workers=args.workers, options=bopts)
     t0 = time.perf_counter()
     got = 0
     fails = 0
@@ -212,7 +222,7 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
     for k in range(args.steps):
         out = []
         for ci, ch in enumerate(chunks):
-            out += jd.pjd_batch(ch, workers=args.workers)
+            out += jd.pjd_batch(ch, workers=args.workers, options=bopts)
             print(f"#   chunk {ci}: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
                   flush=True)
         got += sum(r.neig for r in out)
@@ -232,11 +242,63 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
         "dtype": "f32 factor + f64 inner/outer", "data": "synthetic",
         "config": {"workload": args.workload, "instances": count, "total_n": ntot,
                    "neig": neig, "tol": args.tol, "isearch": isearch, "sigma": sigma,
-                   "workers": args.workers,
+                   "workers": args.workers, "options": bopts,
                    "precond": f"multilevel ILDL droptol {args.droptol}, FP32 (host factor "
                               f"per instance inside the timed region)"},
         "roofline": None, "cpu_baseline": None,
         "result": {"failed_items": fails, "eigenpairs": got, "failed": failed}}))
+
+
+def bench_complex(args, params, neig, madspace, isearch, sigma, mem):
+    """Complex Hermitian throughput through the legacy ZPJD: one set-up call
+    (host factor of the real form + solve, untimed), then `steps` solves
+    with the factor resident (ICNTL(2) = -1)."""
+    from jadamilu_amd import matrices as M
+    from jadamilu_amd import zjd
+    L, W, flux, seed = params
+    ia, ja, a = M.magnetic_anderson(L, W, flux, seed)
+    n = ia.size - 1
+    for kv in args.set:
+        name, value = kv.split("=", 1)
+        zjd.set_option(name, float(value))
+    kw = dict(neig=neig, isearch=isearch, sigma=sigma, madspace=madspace, maxiter=args.maxiter,
+              tol=args.tol, droptol=args.droptol, mem=mem)
+    t0 = time.perf_counter()
+    r0 = zjd.zpjd_host(n, a, ja, ia, icntl=(0, 1, 0, 0, 0), **kw)
+    setup_s = time.perf_counter() - t0
+    print(f"# complex set-up solve: {setup_s:.1f} s, {r0.iter} MATVECs, info {r0.info}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        zjd.zpjd_host(n, a, ja, ia, icntl=(0, -1, 0, 0, 0), **kw)
+    t0 = time.perf_counter()
+    rs = [zjd.zpjd_host(n, a, ja, ia, icntl=(0, -1, 0, 0, 0), **kw) for _ in range(args.steps)]
+    el = time.perf_counter() - t0
+    for k, r in enumerate(rs):
+        print(f"# complex timed solve {k}: {r.iter} MATVECs, info {r.info}", file=sys.stderr,
+              flush=True)
+    r = rs[-1]
+    anorm = float(np.max(np.bincount(np.concatenate([np.repeat(np.arange(n), np.diff(ia)),
+                                                     ja - 1]),
+                                     weights=np.abs(np.concatenate([a, a])), minlength=n)))
+    got = sum(x.neig for x in rs if x.info == 0)
+    zjd.cleanup()
+    print(json.dumps({
+        "metric": "eigenpairs/s to rel. residual 1e-10 (mixed vs FP64 GPU vs CPU); SpMV GB/s",
+        "value": got / el, "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / max(args.steps, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128 as its real form (f64)", "data": "synthetic",
+        "config": {"workload": args.workload, "n": n, "real_form_n": 2 * n,
+                   "nnz_upper": int(ia[-1] - 1), "neig": neig, "tol": args.tol,
+                   "isearch": isearch, "sigma": sigma, "madspace": madspace, "mem": mem,
+                   "flux": flux, "W": W},
+        "roofline": None, "cpu_baseline": None,
+        "result": {"neig": r.neig, "info": r.info, "matvecs": r.iter,
+                   "eigs": [float(v) for v in r.eigs],
+                   "max_res": float(np.max(r.res)) if r.neig else None,
+                   "max_rel_res": float(np.max(r.res)) / anorm if r.neig else None,
+                   "set_up_solve_s": setup_s, "set_up_matvecs": r0.iter,
+                   "failed_timed_solves": sum(x.info != 0 for x in rs)}}))
 
 
 def bench_adjoint(args, params, neig, madspace, isearch, sigma, mem):
@@ -457,6 +519,9 @@ def main():
         return
     if kind == "adjoint":
         bench_adjoint(args, params, neig, madspace, isearch, sigma, mem)
+        return
+    if kind == "complex":
+        bench_complex(args, params, neig, madspace, isearch, sigma, mem)
         return
     ia, ja, a = build_matrix(kind, params)
     n = ia.size - 1

@@ -100,6 +100,87 @@ void djdrevcom_gep(int *n, double *eigs, double *res, double *x, int *lx, int *n
                    double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
                    int *info, double *gap);
 
+/* ---- complex Hermitian drivers (host arrays, their own global context) ---
+ * Each replaces the reference routine of the same name (argument meaning
+ * Doc/ZPJD_prototypes.f; prototypes jadamilu1.h:62-108):
+ *   zpjd_           ZPJD            (jadamilu1.h:62-68)
+ *   zpjdrevcom_     ZPJDREVCOM      (jadamilu1.h:69-77)
+ *   zjdrevcom_      ZJDREVCOM       (jadamilu1.h:79-84)
+ *   zpjdcleanup_    ZPJDCLEANUP     (jadamilu1.h:78)
+ *   zpjd_gep_       ZPJD_GEP        (jadamilu1.h:85-92)
+ *   zpjdrevcom_gep_ ZPJDREVCOM_GEP  (jadamilu1.h:93-103)
+ *   zjdrevcom_gep_  ZJDREVCOM_GEP   (jadamilu1.h:104-109)
+ * A and X (and B) are DOUBLE COMPLEX arrays passed as double *: each entry
+ * is a (real, imaginary) pair of doubles, so X has 2*LX doubles.  The
+ * matrices are Hermitian (B positive definite), given as 1-based CSR of one
+ * triangle with every (real) diagonal entry, or JA(1) < 0 for a diagonal
+ * matrix.  The solve runs on the real symmetric form of order 2N (each entry
+ * a + ib as the block [[a, -b], [b, a]], the complex array's own memory
+ * layout), for 2*NEIG eigenpairs, keeping one vector per complex direction;
+ * reverse-communication requests use the slots X(1:N) (vector in),
+ * X(N+1:2N) (A times it) and X(2N+1:3N) (B times it), and for IJOB = 2 the
+ * right-hand side X(N+1:2N) and the solution X(1:N).  NINIT is capped at
+ * NEIG.  The un-underscored names are exported as aliases. */
+void zpjd_(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x,
+           int *lx, int *neig, double *sigma, int *isearch, int *ninit, int *madspace,
+           int *iter, double *tol, double *shift, double *droptol, double *mem, int *icntl,
+           int *iprint, int *info, double *gap);
+void zpjdrevcom_(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x,
+                 int *lx, int *neig, double *sigma, int *isearch, int *ninit, int *madspace,
+                 int *iter, double *tol, double *shift, double *droptol, double *mem,
+                 int *icntl, int *ijob, int *ndx1, int *ndx2, int *iprint, int *info,
+                 double *gap);
+void zjdrevcom_(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                double *sigma, int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                int *ijob, int *ndx1, int *ndx2, int *iprint, int *info, double *gap);
+void zpjdcleanup_(void);
+void zpjd_gep_(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib, double *eigs,
+               double *res, double *x, int *lx, int *neig, double *sigma, int *isearch,
+               int *ninit, int *madspace, int *iter, double *tol, double *shift,
+               double *droptol, double *mem, int *icntl, int *iprint, int *info, double *gap);
+void zpjdrevcom_gep_(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib,
+                     double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+                     int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                     double *shift, double *droptol, double *mem, int *icntl, int *ijob,
+                     int *ndx1, int *ndx2, int *ndx3, int *iprint, int *info, double *gap);
+void zjdrevcom_gep_(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                    double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+                    double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
+                    int *info, double *gap);
+/* The real symmetric form of order 2n that the complex drivers solve, for a
+ * 1-based one-triangle Hermitian CSR (a: n or nnz complex values as
+ * interleaved doubles; ja[0] < 0: diagonal).  Writes ia2 (2n+1), ja2 and a2
+ * (1-based) when cap is at least the entry count; returns the entry count
+ * (-1 on bad input).  No reference counterpart: exposed for checking. */
+long jdhip_hermitian_real_form(int n, const double *a, const int *ja, const int *ia, int *ia2,
+                               int *ja2, double *a2, long cap);
+void zpjd(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x, int *lx,
+          int *neig, double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+          double *tol, double *shift, double *droptol, double *mem, int *icntl, int *iprint,
+          int *info, double *gap);
+void zpjdrevcom(int *n, double *a, int *ja, int *ia, double *eigs, double *res, double *x,
+                int *lx, int *neig, double *sigma, int *isearch, int *ninit, int *madspace,
+                int *iter, double *tol, double *shift, double *droptol, double *mem,
+                int *icntl, int *ijob, int *ndx1, int *ndx2, int *iprint, int *info,
+                double *gap);
+void zjdrevcom(int *n, double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+               int *isearch, int *ninit, int *madspace, int *iter, double *tol, int *ijob,
+               int *ndx1, int *ndx2, int *iprint, int *info, double *gap);
+void zpjdcleanup(void);
+void zpjd_gep(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib, double *eigs,
+              double *res, double *x, int *lx, int *neig, double *sigma, int *isearch,
+              int *ninit, int *madspace, int *iter, double *tol, double *shift,
+              double *droptol, double *mem, int *icntl, int *iprint, int *info, double *gap);
+void zpjdrevcom_gep(int *n, double *a, int *ja, int *ia, double *b, int *jb, int *ib,
+                    double *eigs, double *res, double *x, int *lx, int *neig, double *sigma,
+                    int *isearch, int *ninit, int *madspace, int *iter, double *tol,
+                    double *shift, double *droptol, double *mem, int *icntl, int *ijob,
+                    int *ndx1, int *ndx2, int *ndx3, int *iprint, int *info, double *gap);
+void zjdrevcom_gep(int *n, double *eigs, double *res, double *x, int *lx, int *neig,
+                   double *sigma, int *isearch, int *ninit, int *madspace, int *iter,
+                   double *tol, int *ijob, int *ndx1, int *ndx2, int *ndx3, int *iprint,
+                   int *info, double *gap);
+
 /* ---- context API: device-resident X ------------------------------------- */
 typedef struct jdhip_ctx jdhip_ctx;
 
@@ -171,6 +252,10 @@ int jdhip_reset(jdhip_ctx *ctx);
 /* The process-global context behind the legacy symbols (created on first
  * use on device $JDHIP_DEVICE, default 0). */
 jdhip_ctx *jdhip_legacy_context(void);
+/* The context behind the complex drivers (created on first use on device
+ * $JDHIP_DEVICE, default 0), e.g. for jdhip_set_option.  No reference
+ * counterpart. */
+jdhip_ctx *jdhip_complex_context(void);
 
 /* HIP stream (as void*) the context launches on; callers servicing IJOB
  * with their own kernels should use it (or synchronise with it). */
@@ -200,6 +285,13 @@ typedef struct {
  * per call).  Returns the number of items with INFO != 0, or -1 on invalid
  * arguments. */
 int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers);
+/* jdhip_pjd_batch with options (jdhip_set_option names and values) applied
+ * to every worker context after the batch's own settings, e.g.
+ * "ildl_graph" 1 (the apply replayed from captured HIP graphs: the same
+ * kernels in the same order, so the same results).  An option that
+ * jdhip_set_option rejects leaves every item with INFO = -54330. */
+int jdhip_pjd_batch_opts(int device, int nitems, jdhip_batch_item *items, int nworkers,
+                         int nopts, const char *const *names, const double *values);
 
 /* ---- symmetric SpMV (MATVECA, JDaux.F:3397-3418) ------------------------- */
 /* Upload a 1-based upper-triangle CSR (diagonal anywhere in each row) as the
@@ -248,7 +340,7 @@ int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, l
  * level-scheduled triangular solves, k > 0 = k Jacobi sweeps per triangular
  * solve, -1 = the default: chosen at set-up by timing both on the device,
  * sweeps only where they reproduce the exact apply to "ildl_tune_tol", by
- * default 3e-2 for definite and 1e-4 for indefinite factors),
+ * default 3e-2 for definite and 1e-2 for indefinite factors),
  * "ildl_autotune" (1 = kernel form of each factor product chosen by timing,
  * the default), "ildl_form" (-1 = automatic; 0, 1, 4, 8, 16, 32, 64 = that form for
  * every product), "ildl_ordering" (0 natural, 1 reverse Cuthill-McKee,

@@ -66,6 +66,13 @@ SIGNATURES = {
     "djdrevcom_gep_": (None, [_i, _d, _d, _d, _i, _i, _d, _i, _i, _i, _i, _d, _i, _i, _i, _i,
                               _i, _i, _d]),
     "dpjd_gep": (None, None), "dpjdrevcom_gep": (None, None), "djdrevcom_gep": (None, None),
+    # complex Hermitian drivers: the same argument types, DOUBLE COMPLEX arrays
+    # as interleaved doubles
+    "zpjd_": None, "zpjdrevcom_": None, "zjdrevcom_": None, "zpjdcleanup_": (None, []),
+    "zpjd_gep_": None, "zpjdrevcom_gep_": None, "zjdrevcom_gep_": None,
+    "zpjd": (None, None), "zpjdrevcom": (None, None), "zjdrevcom": (None, None),
+    "zpjdcleanup": (None, []),
+    "zpjd_gep": (None, None), "zpjdrevcom_gep": (None, None), "zjdrevcom_gep": (None, None),
     "jdhip_create": (_v, [C.c_int]),
     "jdhip_destroy": (None, [_v]),
     "jdhip_pjd": (None, [_v, C.c_int, _d, _i, _i, _d, _d, _v, C.c_long, _i, C.c_double, C.c_int,
@@ -126,7 +133,14 @@ SIGNATURES = {
     "jdhip_adjoint_diag": (C.c_int, [_v, C.c_int, _v, C.c_long, C.c_int, _d, _v]),
     "jdhip_last_error": (C.c_char_p, [_v]),
     "jdhip_pjd_batch": (C.c_int, [C.c_int, C.c_int, C.POINTER(JdhipBatchItem), C.c_int]),
+    "jdhip_pjd_batch_opts": (C.c_int, [C.c_int, C.c_int, C.POINTER(JdhipBatchItem), C.c_int,
+                                       C.c_int, C.POINTER(C.c_char_p), _d]),
+    "jdhip_complex_context": (_v, []),
+    "jdhip_hermitian_real_form": (C.c_long, [C.c_int, _d, _i, _i, _i, _i, _d, C.c_long]),
 }
+for _z in ("zpjd_", "zpjdrevcom_", "zjdrevcom_", "zpjd_gep_", "zpjdrevcom_gep_",
+           "zjdrevcom_gep_"):
+    SIGNATURES[_z] = SIGNATURES["d" + _z[1:]]
 
 
 def lib():

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JDHIP_ARCH", "gfx950")
 
 SOURCES = ["kernels_blas.hip", "kernels_inner.hip", "kernels_mfma.hip", "kernels_spmm.hip", "noalloc.hip", "ildl_kernels.hip", "engine.cpp",
-           "dense.cpp", "driver.cpp", "ildl.cpp", "ildl_factor.cpp", "ordering.cpp", "matching.cpp", "batch.cpp", "persist.cpp",
+           "dense.cpp", "driver.cpp", "ildl.cpp", "ildl_factor.cpp", "ordering.cpp", "matching.cpp", "batch.cpp", "persist.cpp", "complex.cpp",
            "adjoint.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",

@@ -30,8 +30,11 @@
 
 #include "../../include/jadamilu_amd.h"
 
-extern "C" int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers) {
+extern "C" int jdhip_pjd_batch_opts(int device, int nitems, jdhip_batch_item *items,
+                                    int nworkers, int nopts, const char *const *names,
+                                    const double *values) {
     if (nitems < 0 || (nitems > 0 && !items)) return -1;
+    if (nopts < 0 || (nopts > 0 && (!names || !values))) return -1;
     if (nitems == 0) return 0;
     for (int i = 0; i < nitems; ++i) {
         const jdhip_batch_item &it = items[i];
@@ -59,6 +62,13 @@ extern "C" int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, 
         jdhip_set_option(ctx, "ildl_autotune", 0);
         jdhip_set_option(ctx, "ildl_tune_time", 0);
         jdhip_set_option(ctx, "cost_clock", 1);
+        // the caller's options, applied to every worker context; an unknown
+        // name or a value out of range leaves the items unsolved
+        for (int o = 0; o < nopts; ++o)
+            if (jdhip_set_option(ctx, names[o], values[o]) != 0) {
+                jdhip_destroy(ctx);
+                return;
+            }
         double *x = nullptr;
         long xcap = 0;
         std::vector<int> ia, ja;
@@ -119,4 +129,8 @@ extern "C" int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, 
     int failed = 0;
     for (int i = 0; i < nitems; ++i) failed += items[i].info != 0;
     return failed;
+}
+
+extern "C" int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers) {
+    return jdhip_pjd_batch_opts(device, nitems, items, nworkers, 0, nullptr, nullptr);
 }

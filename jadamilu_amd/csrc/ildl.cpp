@@ -1050,16 +1050,16 @@ void IldlPrecond::attach(const IldlHost &h, int sweeps) {
     // closely enough and are faster than the level-scheduled solve.  3 % is
     // enough for definite problems (128^3 Laplacian: 849 MATVECs against 842
     // with uniform sweeps); interior eigenvalues of an indefinite matrix need
-    // the preconditioner much closer to exact along the near-null space of
+    // the preconditioner closer to exact along the near-null space of
     // A - sigma I (Anderson L=100: per-level counts accurate to 3 % on a
-    // random vector stalled the iteration), so there the bound is 1e-4.
-    // (1e-3 runs Anderson L=100 with a quarter fewer sweeps in as many
-    // MATVECs, but on a 10^3 Anderson matrix the iteration then returned
-    // the 2nd-6th eigenvalues nearest sigma instead of the 1st-5th.)
+    // random vector stalled the iteration), so there the bound is 1 %.
+    // Anderson L=100, 10 eigenvalues nearest 0: 1e-4 takes 13-18 sweeps per
+    // solve and 1317 MATVECs, 1e-3 12-16 sweeps and 1217, 1e-2 9-12 sweeps
+    // and 1202 MATVECs (10.3 s, 7.8 s, 6.0 s), the same eigenvalues.
     if (force_form >= 0) impl_->set_forms(force_form);
     else if (autotune) impl_->autotune_forms();
     if (sweeps < 0)
-        impl_->tune_sweeps(tune_tol > 0.0 ? tune_tol : (h.spd ? 0.03 : 1e-4), tune_time);
+        impl_->tune_sweeps(tune_tol > 0.0 ? tune_tol : (h.spd ? 0.03 : 1e-2), tune_time);
     result.sweeps = impl_->sweeps();
     impl_->set_graphs(graphs);
 }

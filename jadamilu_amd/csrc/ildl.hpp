@@ -109,7 +109,7 @@ class IldlPrecond {
     int vec32 = -1;
     bool autotune = true;  // kernel forms by measurement (else by row length)
     int force_form = -1;   // >= 0: this form for every product (tests, experiments)
-    double tune_tol = 0.0;  // sweep tuning tolerance (0: 3e-2 definite, 1e-4 indefinite)
+    double tune_tol = 0.0;  // sweep tuning tolerance (0: 3e-2 definite, 1e-2 indefinite)
     bool tune_time = true;  // sweeps kept only if faster than exact solves (timed)
 
   private:

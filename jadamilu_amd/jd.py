@@ -229,13 +229,14 @@ def pjdrevcom_gep_host(n, a, ja, ia, b, jb, ib, matvec_a, matvec_b, *, neig, ise
     return r
 
 
-def pjd_batch(problems, *, device=0, workers=8, want_x=False):
+def pjd_batch(problems, *, device=0, workers=8, want_x=False, options=None):
     """jdhip_pjd_batch: solve independent problems on one device.
 
     problems: list of dicts with keys ia, ja, a (1-based upper CSR) and
     optionally neig (5), isearch (0), sigma (0.0), madspace (20), maxiter
     (1000), tol (1e-10), shift (0.0), droptol (1e-3), mem (20.0), icntl
-    ((0, 1, 0, 0, 0)).  Returns a list of JDResult (x only if want_x)."""
+    ((0, 1, 0, 0, 0)).  options: {name: value} for every worker context
+    (jdhip_pjd_batch_opts).  Returns a list of JDResult (x only if want_x)."""
     from . import JdhipBatchItem
     L = lib()
     items = (JdhipBatchItem * len(problems))()
@@ -260,7 +261,11 @@ def pjd_batch(problems, *, device=0, workers=8, want_x=False):
             it.icntl[c] = int(v)
         it.eigs, it.res = _p(eigs), _p(res)
         it.x = _p(x) if want_x else C.cast(None, C.POINTER(C.c_double))
-    rc = L.jdhip_pjd_batch(device, len(problems), items, workers)
+    opts = dict(options or {})
+    names = (C.c_char_p * max(len(opts), 1))(*[k.encode() for k in opts])
+    vals = np.array([float(v) for v in opts.values()] or [0.0])
+    rc = L.jdhip_pjd_batch_opts(device, len(problems), items, workers, len(opts), names,
+                                _p(vals))
     if rc < 0:
         raise ValueError("jdhip_pjd_batch: invalid arguments")
     out = []

@@ -88,3 +88,33 @@ def to_scipy(ia, ja, a):
 
 def to_dense(ia, ja, a):
     return to_scipy(ia, ja, a).toarray()
+
+
+def magnetic_anderson(L, W, flux, seed):
+    """Periodic 3D Anderson model L^3 in a magnetic field along z (Landau
+    gauge): hopping -1 along x and z, -exp(2 pi i flux x) along y, diagonal
+    V_i = W (u_i - 1/2) as in anderson().  A complex Hermitian matrix in the
+    input format of the complex drivers: 1-based upper CSR, complex values,
+    diagonal first in each row (flux * L should be an integer so that the
+    phases close around the torus)."""
+    n = L ** 3
+    idx = np.arange(n, dtype=np.int64)
+    x = idx % L
+    rows, cols, vals = [], [], []
+    for k, stride in enumerate((1, L, L * L)):
+        c = (idx // stride) % L
+        nb = np.where(c + 1 < L, idx + stride, idx - (L - 1) * stride)
+        h = -np.exp(2j * np.pi * flux * x) if k == 1 else -np.ones(n, dtype=np.complex128)
+        up = nb > idx  # store the (row < col) entry: H_ij, or conj(H_ji)
+        rows += [idx[up], nb[~up]]
+        cols += [nb[up], idx[~up]]
+        vals += [h[up], np.conj(h[~up])]
+    rows = np.concatenate([idx] + rows)
+    cols = np.concatenate([idx] + cols)
+    vals = np.concatenate([anderson_potential(L, W, seed).astype(np.complex128)] + vals)
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    ia = np.zeros(n + 1, np.int64)
+    np.add.at(ia, rows + 1, 1)
+    ia = np.cumsum(ia)
+    return (ia + 1).astype(np.int32), (cols + 1).astype(np.int32), vals

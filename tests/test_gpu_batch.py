@@ -84,3 +84,25 @@ def test_batch_rejects_bad_items():
     ia, ja, a = M.anderson(6, 16.5, 1)
     with pytest.raises(ValueError):
         jd.pjd_batch([dict(ia=ia, ja=ja, a=a, neig=0)])
+
+
+def test_batch_options_graph_replay_gives_identical_results():
+    """jdhip_pjd_batch_opts with ildl_graph = 1: every apply is replayed from
+    a captured HIP graph (the same kernels in the same order), so every item
+    has the same MATVEC count and bit-identical eigenvalues as without."""
+    from jadamilu_amd import jd
+    _dev()
+    probs = _problems(6, 77, 1)
+    plain = jd.pjd_batch(probs, workers=3)
+    graph = jd.pjd_batch(probs, workers=3, options={"ildl_graph": 1})
+    for r0, r1 in zip(plain, graph):
+        assert r0.info == 0 and r1.info == 0
+        assert r0.iter == r1.iter and r0.neig == r1.neig
+        assert np.array_equal(r0.eigs, r1.eigs)
+
+
+def test_batch_options_rejects_unknown_name():
+    from jadamilu_amd import jd
+    _dev()
+    out = jd.pjd_batch(_problems(2, 5, 0), workers=2, options={"no_such_option": 1})
+    assert all(r.info == -54330 for r in out)
