@@ -243,6 +243,7 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
         "config": {"workload": args.workload, "instances": count, "total_n": ntot,
                    "neig": neig, "tol": args.tol, "isearch": isearch, "sigma": sigma,
                    "workers": args.workers, "options": bopts,
+                   "hw_queues": args.hw_queues_set or None,
                    "precond": f"multilevel ILDL droptol {args.droptol}, FP32 (host factor "
                               f"per instance inside the timed region)"},
         "roofline": None, "cpu_baseline": None,
@@ -495,6 +496,9 @@ def main():
                     help="extra jdhip_set_option(NAME, VALUE), repeatable")
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts of the batched workload")
+    ap.add_argument("--hw-queues", type=int, default=-1,
+                    help="GPU_MAX_HW_QUEUES for this process (-1: one per worker, at most 16, "
+                         "for the batched workload; unset for the others)")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the FP64-against-mixed block on the anderson_100 workload")
     ap.add_argument("--kernels-only", action="store_true",
@@ -508,10 +512,19 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    kind, params, neig, madspace, isearch, sigma, mem = WORKLOADS[args.workload]
+    # hardware queues of this process, fixed when the HIP runtime starts: the
+    # batched workload's worker streams run concurrently only on separate
+    # queues (the runtime's default is 4)
+    hwq = args.hw_queues if args.hw_queues >= 0 else (min(args.workers, 16)
+                                                      if kind == "batch" else 0)
+    if hwq > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(hwq, 32))
+    args.hw_queues_set = hwq
+
     import torch
     from jadamilu_amd.jd import Context, lx_optimal
 
-    kind, params, neig, madspace, isearch, sigma, mem = WORKLOADS[args.workload]
     if args.mem > 0:
         mem = args.mem
     if kind == "batch":
