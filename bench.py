@@ -193,6 +193,8 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
     count, lmin, lmax, wmin, wmax, seed = params
     rng = np.random.default_rng(seed)
     probs, meta, ntot = [], [], 0
+    if args.batch_count > 0:  # the first instances of the same sequence only
+        count = min(count, args.batch_count)
     for _ in range(count):
         L = int(rng.integers(lmin, lmax + 1))
         W = float(rng.uniform(wmin, wmax))
@@ -203,9 +205,7 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
                           madspace=madspace, maxiter=args.maxiter, tol=args.tol,
                           droptol=args.droptol, mem=mem, icntl=(0, 0, 0, 0, 1)))
     print(f"# batch of {count} problems, total n {ntot}", file=sys.stderr, flush=True)
-    bopts = {}
-    if args.graph:
-        bopts["ildl_graph"] = 1
+    bopts = {}  # (graph replay is not offered by the batch driver)
     for kv in args.set:
         name, value = kv.split("=", 1)
         bopts[name] = float(value)
@@ -494,11 +494,13 @@ def main():
                     help="replay the preconditioner apply from captured HIP graphs")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="extra jdhip_set_option(NAME, VALUE), repeatable")
+    ap.add_argument("--batch-count", type=int, default=0,
+                    help="batched workload: solve only the first N instances (0: all), for "
+                         "profiles")
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts of the batched workload")
-    ap.add_argument("--hw-queues", type=int, default=-1,
-                    help="GPU_MAX_HW_QUEUES for this process (-1: one per worker, at most 16, "
-                         "for the batched workload; unset for the others)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (0: the runtime's default)")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the FP64-against-mixed block on the anderson_100 workload")
     ap.add_argument("--kernels-only", action="store_true",
@@ -513,11 +515,10 @@ def main():
         sys.exit(2)
 
     kind, params, neig, madspace, isearch, sigma, mem = WORKLOADS[args.workload]
-    # hardware queues of this process, fixed when the HIP runtime starts: the
-    # batched workload's worker streams run concurrently only on separate
-    # queues (the runtime's default is 4)
-    hwq = args.hw_queues if args.hw_queues >= 0 else (min(args.workers, 16)
-                                                      if kind == "batch" else 0)
+    # hardware queues of this process, fixed when the HIP runtime starts (the
+    # runtime's default, 4, when not given: 16 made the batched workload
+    # slower)
+    hwq = max(args.hw_queues, 0)
     if hwq > 0:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(hwq, 32))
     args.hw_queues_set = hwq

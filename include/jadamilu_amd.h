@@ -287,8 +287,9 @@ typedef struct {
 int jdhip_pjd_batch(int device, int nitems, jdhip_batch_item *items, int nworkers);
 /* jdhip_pjd_batch with options (jdhip_set_option names and values) applied
  * to every worker context after the batch's own settings, e.g.
- * "ildl_graph" 1 (the apply replayed from captured HIP graphs: the same
- * kernels in the same order, so the same results).  An option that
+ * "ildl_sweeps" or "ildl_tune_tol".  "ildl_graph" 1 is refused (-1): while
+ * one worker captures a graph, another worker's allocations may
+ * synchronise the device and invalidate the capture.  An option that
  * jdhip_set_option rejects leaves every item with INFO = -54330. */
 int jdhip_pjd_batch_opts(int device, int nitems, jdhip_batch_item *items, int nworkers,
                          int nopts, const char *const *names, const double *values);

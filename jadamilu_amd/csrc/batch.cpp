@@ -25,6 +25,7 @@
 #include <exception>
 #include <mutex>
 #include <numeric>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -35,6 +36,12 @@ extern "C" int jdhip_pjd_batch_opts(int device, int nitems, jdhip_batch_item *it
                                     const double *values) {
     if (nitems < 0 || (nitems > 0 && !items)) return -1;
     if (nopts < 0 || (nopts > 0 && (!names || !values))) return -1;
+    // graph capture of the preconditioner apply is not offered here: while
+    // one worker captures, another worker's allocations and frees (factor
+    // uploads, workspaces) may synchronise the device and invalidate the
+    // capture
+    for (int o = 0; o < nopts; ++o)
+        if (names[o] && std::string(names[o]) == "ildl_graph" && values[o] != 0.0) return -1;
     if (nitems == 0) return 0;
     for (int i = 0; i < nitems; ++i) {
         const jdhip_batch_item &it = items[i];

@@ -481,6 +481,9 @@ void zdrive(Kind kind, bool gep, int *n, double *a, int *ja, int *ia, double *b,
         }
         finish(Z, kind == kPjd ? a : nullptr, ja, ia, eigs, res, x, &Z.neig2);
         *neig = Z.neig2;
+        // the products ran out with one copy of the last direction missing:
+        // every wanted complex eigenpair was found
+        if (*info > 0 && Z.neig2 == Z.nwant) *info = 0;
         if (ijob) *ijob = 0;
     } catch (const std::exception &) {
         *info = JDHIP_INFO_DEVICE_ERROR;

@@ -242,6 +242,15 @@ void SolverOps::bmatvec(const double *x, double *y) {
 // Full (both triangles) CSR of the symmetric matrix, built on the host from
 // the upper triangle and uploaded once: the SpMV then reads each row with
 // coalesced loads and needs no atomics.
+// Host-to-device copy on the context's stream, waited for (the source is a
+// temporary).  The legacy null stream is never used: an operation there
+// would wait for the streams of every other context in the process (the
+// batch workers') and would break another thread's graph capture.
+static void h2d(hipStream_t s, void *dst, const void *src, size_t bytes) {
+    JDHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    JDHIP_CHECK(hipStreamSynchronize(s));
+}
+
 void Solver::upload_csr(CsrDev &A, int n_, const int *ia_, const int *ja_, const double *a_,
                         int lpr) {
     std::vector<int> cnt(n_ + 1, 0);
@@ -274,14 +283,14 @@ void Solver::upload_csr(CsrDev &A, int n_, const int *ia_, const int *ja_, const
     JDHIP_CHECK(hipMalloc(&A.rowptr, sizeof(int) * (n_ + 1)));
     JDHIP_CHECK(hipMalloc(&A.col, sizeof(int) * std::max(nnz, 1L)));
     JDHIP_CHECK(hipMalloc(&A.val, sizeof(double) * std::max(nnz, 1L)));
-    JDHIP_CHECK(hipMemcpy(A.rowptr, cnt.data(), sizeof(int) * (n_ + 1), hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.col, col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    h2d(ctx.stream, A.rowptr, cnt.data(), sizeof(int) * (n_ + 1));
+    h2d(ctx.stream, A.col, col.data(), sizeof(int) * nnz);
+    h2d(ctx.stream, A.val, val.data(), sizeof(double) * nnz);
     {
         // FP32 copy of the values for the mixed-precision inner solver
         std::vector<float> vf(val.begin(), val.end());
         JDHIP_CHECK(hipMalloc(&A.valf, sizeof(float) * std::max(nnz, 1L)));
-        JDHIP_CHECK(hipMemcpy(A.valf, vf.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+        h2d(ctx.stream, A.valf, vf.data(), sizeof(float) * nnz);
     }
     A.n = n_;
     A.nnz = nnz;
@@ -344,12 +353,12 @@ void Solver::build_sell(CsrDev &A, int n_, const std::vector<int> &rp,
     JDHIP_CHECK(hipMalloc(&A.s_val, sizeof(double) * std::max(ent, 1L)));
     JDHIP_CHECK(hipMalloc(&A.s_valf, sizeof(float) * std::max(ent, 1L)));
     JDHIP_CHECK(hipMalloc(&A.s_idx, sidx.size()));
-    JDHIP_CHECK(hipMemcpy(A.s_vptr, vptr.data(), sizeof(long) * (ns + 1), hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.s_iptr, iptr.data(), sizeof(long) * (ns + 1), hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.s_wm, wm.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.s_val, sv.data(), sizeof(double) * ent, hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.s_valf, svf.data(), sizeof(float) * ent, hipMemcpyHostToDevice));
-    JDHIP_CHECK(hipMemcpy(A.s_idx, sidx.data(), sidx.size(), hipMemcpyHostToDevice));
+    h2d(ctx.stream, A.s_vptr, vptr.data(), sizeof(long) * (ns + 1));
+    h2d(ctx.stream, A.s_iptr, iptr.data(), sizeof(long) * (ns + 1));
+    h2d(ctx.stream, A.s_wm, wm.data(), sizeof(int) * ns);
+    h2d(ctx.stream, A.s_val, sv.data(), sizeof(double) * ent);
+    h2d(ctx.stream, A.s_valf, svf.data(), sizeof(float) * ent);
+    h2d(ctx.stream, A.s_idx, sidx.data(), sidx.size());
     A.nslices = ns;
     A.s_entries = ent;
     A.s_idx_bytes = iptr[ns];

@@ -28,12 +28,13 @@ void DevCtx::init(int dev) {
     JDHIP_CHECK(hipStreamCreate(&stream));
     JDHIP_CHECK(hipMalloc(&rb.partials, sizeof(double) * kMaxPartials));
     JDHIP_CHECK(hipMalloc(&rb.counter, sizeof(unsigned) * 16));
-    JDHIP_CHECK(hipMemset(rb.counter, 0, sizeof(unsigned) * 16));
+    JDHIP_CHECK(hipMemsetAsync(rb.counter, 0, sizeof(unsigned) * 16, stream));
     JDHIP_CHECK(hipMalloc(&d_small, sizeof(double) * 8192));
-    JDHIP_CHECK(hipMemset(d_small, 0, sizeof(double) * 8192));
+    JDHIP_CHECK(hipMemsetAsync(d_small, 0, sizeof(double) * 8192, stream));
     rb.out = d_small;
     JDHIP_CHECK(hipMalloc(&d_sc, sizeof(InnerScalars)));
-    JDHIP_CHECK(hipMemset(d_sc, 0, sizeof(InnerScalars)));
+    JDHIP_CHECK(hipMemsetAsync(d_sc, 0, sizeof(InnerScalars), stream));
+    JDHIP_CHECK(hipStreamSynchronize(stream));
     JDHIP_CHECK(hipHostMalloc(&h_pin, sizeof(double) * 8192, hipHostMallocDefault));
     JDHIP_CHECK(hipHostMalloc(&h_sc, sizeof(InnerScalars), hipHostMallocDefault));
     std::memset(h_sc, 0, sizeof(InnerScalars));
@@ -165,7 +166,9 @@ fail:
                 rng_ ^= rng_ >> 27;
                 r[i] = (double)((rng_ * 2685821657736338717ULL) >> 11) * 0x1.0p-53;
             }
-            JDHIP_CHECK(hipMemcpy(w, r.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            JDHIP_CHECK(hipMemcpyAsync(w, r.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                                       ctx_->stream));
+            JDHIP_CHECK(hipStreamSynchronize(ctx_->stream));
             nrnd = false;
             goto restart;
         }

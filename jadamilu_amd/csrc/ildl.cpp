@@ -22,12 +22,18 @@ namespace jdhip {
 
 namespace {
 
+// Copies run on the context's stream and are waited for (the source is a
+// temporary): the legacy null stream would wait for every other context's
+// stream in the process and break another thread's graph capture.
 template <typename X>
-X *dev_upload(const std::vector<X> &h) {
+X *dev_upload(hipStream_t s, const std::vector<X> &h) {
     X *p = nullptr;
     size_t bytes = sizeof(X) * std::max<size_t>(h.size(), 1);
     JDHIP_CHECK(hipMalloc(&p, bytes));
-    if (!h.empty()) JDHIP_CHECK(hipMemcpy(p, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice));
+    if (!h.empty()) {
+        JDHIP_CHECK(hipMemcpyAsync(p, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice, s));
+        JDHIP_CHECK(hipStreamSynchronize(s));
+    }
     return p;
 }
 
@@ -293,8 +299,8 @@ class IldlImpl : public IldlImplBase {
             D.nC = nC;
             std::vector<int> permN(n);
             for (int t = 0; t < n; ++t) permN[t] = perm[q[t]];
-            D.perm = track(dev_upload(permN));
-            D.scal = track(dev_upload(scal));
+            D.perm = track(dev_upload(ctx_->stream, permN));
+            D.scal = track(dev_upload(ctx_->stream, scal));
             // D^-1 per stored row: 1 / d for a 1x1 pivot; for a 2x2 block
             // [a o; o c] the row's entries of [c -o; -o a] / (ac - o^2) and the
             // partner's storage index
@@ -315,10 +321,10 @@ class IldlImpl : public IldlImplBase {
                     dpair_s[t] = qinv[pb];
                 }
             }
-            D.dinv = track(dev_upload(dinv));
+            D.dinv = track(dev_upload(ctx_->stream, dinv));
             if (any_pair) {
-                D.doinv = track(dev_upload(doinv));
-                D.dpair = track(dev_upload(dpair_s));
+                D.doinv = track(dev_upload(ctx_->stream, doinv));
+                D.dpair = track(dev_upload(ctx_->stream, dpair_s));
             }
             // L_B and L_CB in the storage numbering
             std::vector<int> bp(nB + 1, 0), bi, cp(nC + 1, 0), ci;
@@ -348,15 +354,15 @@ class IldlImpl : public IldlImplBase {
             // lines (the transposes built below inherit the order)
             sort_rows(nB, bp, bi, bv);
             sort_rows(nC, cp, ci, cv);
-            D.lb_ptr = track(dev_upload(bp));
-            D.lb_idx = track(dev_upload(bi));
-            D.lb_val = track(dev_upload(bv));
+            D.lb_ptr = track(dev_upload(ctx_->stream, bp));
+            D.lb_idx = track(dev_upload(ctx_->stream, bi));
+            D.lb_val = track(dev_upload(ctx_->stream, bv));
             std::vector<int> tp, ti;
             std::vector<T> tv;
             transpose(nB, nB, bp, bi, bv, tp, ti, tv);
-            D.lt_ptr = track(dev_upload(tp));
-            D.lt_idx = track(dev_upload(ti));
-            D.lt_val = track(dev_upload(tv));
+            D.lt_ptr = track(dev_upload(ctx_->stream, tp));
+            D.lt_idx = track(dev_upload(ctx_->stream, ti));
+            D.lt_val = track(dev_upload(ctx_->stream, tv));
             upload_row_blocks(nB, bp, D.rb_b, D.nrb_b);
             upload_row_blocks(nB, tp, D.rb_t, D.nrb_t);
             D.sf = upload_sell(build_sell(nB, bp, bi, bv, sigma_));
@@ -367,24 +373,24 @@ class IldlImpl : public IldlImplBase {
                 std::vector<T> otv;
                 schedule(nB, L.lb_ptr, L.lb_idx, true, D.fwd_ptr, rows);
                 for (int &r : rows) r = qinv[r];
-                D.fwd_rows = track(dev_upload(rows));
+                D.fwd_rows = track(dev_upload(ctx_->stream, rows));
                 transpose(nB, nB, L.lb_ptr, L.lb_idx, L.lb_val, otp, oti, otv);
                 schedule(nB, otp, oti, false, D.bwd_ptr, rows);
                 for (int &r : rows) r = qinv[r];
-                D.bwd_rows = track(dev_upload(rows));
+                D.bwd_rows = track(dev_upload(ctx_->stream, rows));
             }
             D.sc = upload_sell(build_sell(nC, cp, ci, cv, sigma_, true));
-            D.lc_ptr = track(dev_upload(cp));
-            D.lc_idx = track(dev_upload(ci));
-            D.lc_val = track(dev_upload(cv));
+            D.lc_ptr = track(dev_upload(ctx_->stream, cp));
+            D.lc_idx = track(dev_upload(ctx_->stream, ci));
+            D.lc_val = track(dev_upload(ctx_->stream, cv));
             upload_row_blocks(nC, cp, D.rb_c, D.nrb_c);
             D.wc = pick_lanes((long)ci.size(), nC);
             D.wf = D.wb = pick_lanes((long)bi.size(), nB);
             transpose(nC, nB, cp, ci, cv, tp, ti, tv);
             D.sct = upload_sell(build_sell(nB, tp, ti, tv, sigma_, true));
-            D.lct_ptr = track(dev_upload(tp));
-            D.lct_idx = track(dev_upload(ti));
-            D.lct_val = track(dev_upload(tv));
+            D.lct_ptr = track(dev_upload(ctx_->stream, tp));
+            D.lct_idx = track(dev_upload(ctx_->stream, ti));
+            D.lct_val = track(dev_upload(ctx_->stream, tv));
             upload_row_blocks(nB, tp, D.rb_ct, D.nrb_ct);
             D.wct = pick_lanes((long)ti.size(), nB);
             D.buf = track(dev_alloc<W>(n));
@@ -415,12 +421,12 @@ class IldlImpl : public IldlImplBase {
                     const int oa = nlev ? qC_last[a] : a, ob = nlev ? qC_last[b] : b;
                     mv[(size_t)a * m + b] = (T)f_.dinv[(size_t)oa * m + ob];
                 }
-            dmat_ = track(dev_upload(mv));
+            dmat_ = track(dev_upload(ctx_->stream, mv));
             dx_ = track(dev_alloc<W>(m));
             dy_ = track(dev_alloc<W>(m));
         }
         for (auto &D : lv_) D.k = sweeps_;
-        JDHIP_CHECK(hipDeviceSynchronize());
+        JDHIP_CHECK(hipStreamSynchronize(ctx_->stream));
     }
 
     // Triangular solve with L_B (fwd) or L_B^T in place on x (nB entries).
@@ -568,19 +574,19 @@ class IldlImpl : public IldlImplBase {
     void upload_row_blocks(int nrows, const std::vector<int> &ptr, int *&dev, int &nblk) {
         const std::vector<int> rb = stream_row_blocks(nrows, ptr.data());
         nblk = (int)rb.size() - 1;
-        dev = track(dev_upload(rb));
+        dev = track(dev_upload(ctx_->stream, rb));
     }
 
     SellDev<T> upload_sell(const SellHost<T> &h) {
         SellDev<T> d;
         d.v.nslices = (int)h.code.size();
-        d.v.vptr = track(dev_upload(h.vptr));
-        d.v.iptr = track(dev_upload(h.iptr));
-        d.v.code = track(dev_upload(h.code));
-        d.v.rowid = track(dev_upload(h.rowid));
-        d.v.cbase = track(dev_upload(h.cbase));
-        d.v.idx = track(dev_upload(h.idx));
-        d.val = track(dev_upload(h.val));
+        d.v.vptr = track(dev_upload(ctx_->stream, h.vptr));
+        d.v.iptr = track(dev_upload(ctx_->stream, h.iptr));
+        d.v.code = track(dev_upload(ctx_->stream, h.code));
+        d.v.rowid = track(dev_upload(ctx_->stream, h.rowid));
+        d.v.cbase = track(dev_upload(ctx_->stream, h.cbase));
+        d.v.idx = track(dev_upload(ctx_->stream, h.idx));
+        d.val = track(dev_upload(ctx_->stream, h.val));
         d.idx_bytes = (long)h.idx.size();
         d.nnz16 = h.nnz16;
         d.nnz32 = h.nnz32;
@@ -611,7 +617,7 @@ class IldlImpl : public IldlImplBase {
             hr[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
         }
         double *r = dev_alloc<double>(n), *z = dev_alloc<double>(n);
-        JDHIP_CHECK(hipMemcpy(r, hr.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        JDHIP_CHECK(hipMemcpyAsync(r, hr.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
         hipEvent_t e0, e1;
         JDHIP_CHECK(hipEventCreate(&e0));
         JDHIP_CHECK(hipEventCreate(&e1));
@@ -639,13 +645,15 @@ class IldlImpl : public IldlImplBase {
         } else {
             apply(z, r);
         }
-        JDHIP_CHECK(hipMemcpy(h0.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost));
+        JDHIP_CHECK(hipMemcpyAsync(h0.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        JDHIP_CHECK(hipStreamSynchronize(s));
         double n0 = 0.0;
         for (double v : h0) n0 += v * v;
         n0 = std::sqrt(n0);
         auto close = [&]() {  // apply of the current setting within tol of exact?
             apply(z, r);
-            JDHIP_CHECK(hipMemcpy(hk.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost));
+            JDHIP_CHECK(hipMemcpyAsync(hk.data(), z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            JDHIP_CHECK(hipStreamSynchronize(s));
             double e = 0.0;
             for (int i = 0; i < n; ++i) e += (hk[i] - h0[i]) * (hk[i] - h0[i]);
             return std::sqrt(e) <= tol * n0;
@@ -897,9 +905,9 @@ class IldlImpl : public IldlImplBase {
         if (!idp_) {
             std::vector<int> id(f_.m);
             for (int i = 0; i < f_.m; ++i) id[i] = i;
-            idp_ = track(dev_upload(id));
+            idp_ = track(dev_upload(ctx_->stream, id));
             std::vector<double> one(f_.m, 1.0);
-            onep_ = track(dev_upload(one));
+            onep_ = track(dev_upload(ctx_->stream, one));
         }
         return idp_;
     }

@@ -86,19 +86,35 @@ def test_batch_rejects_bad_items():
         jd.pjd_batch([dict(ia=ia, ja=ja, a=a, neig=0)])
 
 
-def test_batch_options_graph_replay_gives_identical_results():
-    """jdhip_pjd_batch_opts with ildl_graph = 1: every apply is replayed from
-    a captured HIP graph (the same kernels in the same order), so every item
-    has the same MATVEC count and bit-identical eigenvalues as without."""
+def test_batch_options_apply_to_every_worker_and_refuse_graphs():
+    """jdhip_pjd_batch_opts: an option given for the batch reaches every
+    worker (exact triangular solves, ildl_sweeps = 0, give the same results
+    as a single-problem call with that option), and graph capture, which
+    another worker's allocations could invalidate, is refused."""
     from jadamilu_amd import jd
+    from jadamilu_amd.jd import Context, lx_optimal
     _dev()
-    probs = _problems(6, 77, 1)
-    plain = jd.pjd_batch(probs, workers=3)
-    graph = jd.pjd_batch(probs, workers=3, options={"ildl_graph": 1})
-    for r0, r1 in zip(plain, graph):
-        assert r0.info == 0 and r1.info == 0
-        assert r0.iter == r1.iter and r0.neig == r1.neig
-        assert np.array_equal(r0.eigs, r1.eigs)
+    probs = _problems(4, 77, 0)
+    out = jd.pjd_batch(probs, workers=2, options={"ildl_sweeps": 0})
+    for pr, r in list(zip(probs, out))[:2]:
+        assert r.info == 0
+        n = pr["ia"].size - 1
+        ctx = Context(0)
+        for name, v in [("ildl_threads", 1), ("ildl_autotune", 0), ("ildl_tune_time", 0),
+                        ("cost_clock", 1), ("ildl_sweeps", 0)]:
+            ctx.set_option(name, v)
+        lx = lx_optimal(n, 4, 25)
+        X = torch.zeros(lx, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        rs = ctx.pjd(n, pr["a"], pr["ja"], pr["ia"], X.data_ptr(), lx, neig=4, isearch=2,
+                     sigma=0.0, madspace=25, maxiter=4000, tol=1e-10, droptol=1e-3, mem=25.0,
+                     icntl=pr["icntl"])
+        torch.cuda.synchronize()
+        ctx.close()
+        assert rs.iter == r.iter
+        np.testing.assert_allclose(np.sort(rs.eigs), np.sort(r.eigs), rtol=1e-12, atol=1e-14)
+    with pytest.raises(ValueError):
+        jd.pjd_batch(probs[:1], workers=1, options={"ildl_graph": 1})
 
 
 def test_batch_options_rejects_unknown_name():
