@@ -214,10 +214,15 @@ def bench_batch(args, params, neig, madspace, isearch, sigma, mem):
     t0 = time.perf_counter()
     got = 0
     fails = 0
-    # submitted in chunks of 64 problems (largest first inside each chunk),
-    # with a progress line per chunk
-    order = sorted(range(count), key=lambda i: -(probs[i]["ia"].size))
-    chunks = [[probs[i] for i in order[c:c + 64]] for c in range(0, count, 64)]
+    # submitted in two chunks with a progress line after each; the instances
+    # are dealt to the chunks in turn by decreasing size so that both hold
+    # the same mix (the batch driver takes each chunk's largest first)
+    nch = 2 if count > 32 else 1
+    bysize = sorted(range(count), key=lambda i: -(probs[i]["ia"].size))
+    order = [i for c in range(nch) for i in bysize[c::nch]]
+    bounds = [0] + [len(bysize[c::nch]) for c in range(nch)]
+    bounds = np.cumsum(bounds)
+    chunks = [[probs[i] for i in order[bounds[c]:bounds[c + 1]]] for c in range(nch)]
     failed = []
     for k in range(args.steps):
         out = []

@@ -893,6 +893,7 @@ int dense_inverse(const SymCsr &S, bool spd, std::vector<double> &inv, int &nneg
         for (int i = 0; i < m; ++i) std::swap(Ar(i, p), Ar(i, q));
         std::swap(perm[p], perm[q]);
     };
+    std::vector<double> f1(m), f2(m);  // the pivot rows' multipliers
     int k = 0;
     while (k < m) {
         double akk = std::fabs(Ar(k, k)), colmax = 0.0;
@@ -917,13 +918,19 @@ int dense_inverse(const SymCsr &S, bool spd, std::vector<double> &inv, int &nneg
         }
         int kk = k + kstep - 1;
         sym_swap(kk, kp);
+        // The trailing updates run row by row over contiguous memory (the
+        // matrix is row-major); every entry receives the same single update
+        // per step as in the column-by-column order, so the result is the
+        // same to the bit.
         if (kstep == 1) {
             double dk = Ar(k, k);
             for (int i = k + 1; i < m; ++i) Ar(i, k) /= dk;
-            for (int j = k + 1; j < m; ++j) {
-                double f = Ar(j, k) * dk;
-                if (f == 0.0) continue;
-                for (int i = k + 1; i < m; ++i) Ar(i, j) -= Ar(i, k) * f;
+            for (int j = k + 1; j < m; ++j) f1[j] = Ar(j, k) * dk;
+            for (int i = k + 1; i < m; ++i) {
+                const double lik = Ar(i, k);
+                double *row = &A[(size_t)i * m];
+                for (int j = k + 1; j < m; ++j)
+                    if (f1[j] != 0.0) row[j] -= lik * f1[j];
             }
             for (int j = k + 1; j < m; ++j) Ar(k, j) = 0.0;
             bsize[k] = 1;
@@ -939,9 +946,13 @@ int dense_inverse(const SymCsr &S, bool spd, std::vector<double> &inv, int &nneg
                 Ar(i, k + 1) = l2;
             }
             for (int j = k + 2; j < m; ++j) {
-                double f1 = Ar(j, k) * d11 + Ar(j, k + 1) * d21;
-                double f2 = Ar(j, k) * d21 + Ar(j, k + 1) * d22;
-                for (int i = k + 2; i < m; ++i) Ar(i, j) -= Ar(i, k) * f1 + Ar(i, k + 1) * f2;
+                f1[j] = Ar(j, k) * d11 + Ar(j, k + 1) * d21;
+                f2[j] = Ar(j, k) * d21 + Ar(j, k + 1) * d22;
+            }
+            for (int i = k + 2; i < m; ++i) {
+                const double a1 = Ar(i, k), a2 = Ar(i, k + 1);
+                double *row = &A[(size_t)i * m];
+                for (int j = k + 2; j < m; ++j) row[j] -= a1 * f1[j] + a2 * f2[j];
             }
             for (int j = k + 2; j < m; ++j) Ar(k, j) = Ar(k + 1, j) = 0.0;
             Ar(k, k + 1) = d21;
@@ -983,17 +994,23 @@ int dense_inverse(const SymCsr &S, bool spd, std::vector<double> &inv, int &nneg
             k += 2;
         }
     }
-    // W = L^-1 (unit lower), then inv_perm = W^T Dinv W
+    // W = L^-1 (unit lower), then inv_perm = W^T Dinv W.  L's columns are
+    // copied to contiguous rows of Lt first (Lt[i][r] = L(r, i)).
+    std::vector<double> Lt((size_t)m * m, 0.0);
+    for (int r = 0; r < m; ++r)
+        for (int i = 0; i < r; ++i) Lt[(size_t)i * m + r] = Ar(r, i);
     std::vector<double> W((size_t)m * m, 0.0);
+    std::vector<double> col(m);
     for (int j = 0; j < m; ++j) {  // column j of L^-1
-        std::vector<double> col(m, 0.0);
+        std::fill(col.begin(), col.end(), 0.0);
         col[j] = 1.0;
         for (int i = j; i < m; ++i) {
             double v = col[i];
             if (v == 0.0) continue;
             int ilo = i + 1;
             if (bsize[i] == 2) ilo = i + 2;  // L(i+1,i) is zero inside a 2x2 block
-            for (int r = ilo; r < m; ++r) col[r] -= Ar(r, i) * v;
+            const double *li = &Lt[(size_t)i * m];
+            for (int r = ilo; r < m; ++r) col[r] -= li[r] * v;
         }
         for (int i = 0; i < m; ++i) W[(size_t)i * m + j] = col[i];
     }
