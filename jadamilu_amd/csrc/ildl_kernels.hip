@@ -222,8 +222,27 @@ __device__ __forceinline__ void stream_rows(long blk, const int *__restrict__ rb
         }
         return;
     }
-    // ne <= kStreamCap and r1 - r0 <= kB by construction of rblk
-    for (int k = tid; k < ne; k += kB) prod[k] = (double)val[e0 + k] * (double)x[idx[e0 + k]];
+    // ne <= kStreamCap and r1 - r0 <= kB by construction of rblk.  A thread
+    // takes the entries tid, tid + kB, ...: all their column indices are
+    // loaded first, then the values and the gathers of x, so that each wave
+    // has up to 3 x kPer loads in flight rather than one index and one
+    // gather at a time.  Positions past the block's entries load its last
+    // entry again (in bounds, branch-free) and are not stored.
+    constexpr int kPer = kStreamCap / kB;
+    static_assert(kPer * kB == kStreamCap, "stream blocks hold kPer entries per thread");
+    if (ne > 0) {
+        const int last = ne - 1;
+        int ci[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) ci[u] = idx[e0 + min(tid + u * kB, last)];
+        double pv[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            pv[u] = (double)val[e0 + min(tid + u * kB, last)] * (double)x[ci[u]];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            if (tid + u * kB < ne) prod[tid + u * kB] = pv[u];
+    }
     __syncthreads();
     const int nr = r1 - r0;
     int t = 64;
