@@ -11,6 +11,7 @@
 // iteration there, while the vector traffic is small next to the
 // factor's and the extra arithmetic is free in these HBM-bound loops.
 #include "ildl_kernels.hpp"
+#include "rowsum.hpp"
 #include "xcd.hpp"
 
 #include <algorithm>
@@ -148,10 +149,7 @@ __global__ __launch_bounds__(kB) void k_jacobi_csr(int nrows, const int *__restr
                                                    const W *xin, W *xout, int xcd) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = logical_block(xcd) * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
-        double s = 0.0;
-#pragma unroll 4
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR)
-            s += (double)val[k] * (double)xin[idx[k]];
+        double s = csr_lane_sum<LPR>(idx, val, xin, ptr[r], ptr[r + 1], l);
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
         if (l == 0) xout[r] = (W)((double)b[r] - s);
@@ -166,9 +164,7 @@ __global__ __launch_bounds__(kB) void k_sub_csr(int nrows, const int *__restrict
                                                 int xcd) {
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
     for (long r = logical_block(xcd) * (kB / LPR) + g; r < nrows; r += (long)gridDim.x * (kB / LPR)) {
-        double s = 0.0;
-#pragma unroll 4
-        for (int k = ptr[r] + l; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[idx[k]];
+        double s = csr_lane_sum<LPR>(idx, val, x, ptr[r], ptr[r + 1], l);
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
         if (l == 0) y[r] = (W)((double)y[r] - s);

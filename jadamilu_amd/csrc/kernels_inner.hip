@@ -7,6 +7,7 @@
 // that turns the rest of the step into no-ops (the reference's early exits).
 #include "kernels.hpp"
 #include "reduce.hpp"
+#include "rowsum.hpp"
 #include "xcd.hpp"
 
 #include <algorithm>
@@ -92,8 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ 
     double acc[1] = {0.0};
     for (long row = logical_block(1) * GPB + g; row < n; row += (long)gridDim.x * GPB) {
         const int b = rowptr[row], e = rowptr[row + 1];
-        double s = 0.0;
-        for (int k = b + l; k < e; k += LPR) s += (double)val[k] * (double)x[col[k]];
+        double s = csr_lane_sum<LPR>(col, val, x, b, e, l);
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
         if (l == 0) {
