@@ -154,6 +154,48 @@ def zpjd_gep_host(n, a, ja, ia, b, jb, ib, *, neig, isearch=0, sigma=0.0, ninit=
     return _result(x, n, eigs, res, NE, IT, INFO, GAP, SH, DT)
 
 
+def zpjdrevcom_gep_host(n, a, ja, ia, b, jb, ib, matvec_a, matvec_b, *, neig, isearch=0,
+                        sigma=0.0, ninit=0, madspace=20, maxiter=1000, tol=1e-10, shift=0.0,
+                        droptol=1e-3, mem=20.0, icntl=(0, 0, 0, 0, 0), lx=None, iprint=0):
+    """ZPJDREVCOM_GEP through `zpjdrevcom_gep_`: the preconditioner is built
+    from (a, ja, ia) and (b, jb, ib); the caller's products serve IJOB 1 (A
+    and B), 3 (A only) and 4 (B only)."""
+    L = lib()
+    lx = lx or lx_optimal_gep(n, neig, madspace)
+    x = np.zeros(lx, dtype=np.complex128)
+    a, ja, ia = _cplx(a), _int(ja), _int(ia)
+    b, jb, ib = _cplx(b), _int(jb), _int(ib)
+    eigs, res = np.zeros(max(neig, 1)), np.zeros(max(neig, 1))
+    ic = _icntl(icntl)
+    N, LX, NE, IS, NI, MS, IT = _i(n), _i(lx), _i(neig), _i(isearch), _i(ninit), _i(madspace), _i(maxiter)
+    SG, TL, SH, DT, ME = _d(sigma), _d(tol), _d(shift), _d(droptol), _d(mem)
+    IJ, N1, N2, N3, IP, INFO, GAP = _i(0), _i(0), _i(0), _i(0), _i(iprint), _i(0), _d(0)
+    counts = {1: 0, 3: 0, 4: 0}
+    while True:
+        L.zpjdrevcom_gep_(C.byref(N), _pd(a), _pi(ja), _pi(ia), _pd(b), _pi(jb), _pi(ib),
+                          _pd(eigs), _pd(res), _pd(x), C.byref(LX), C.byref(NE), C.byref(SG),
+                          C.byref(IS), C.byref(NI), C.byref(MS), C.byref(IT), C.byref(TL),
+                          C.byref(SH), C.byref(DT), C.byref(ME), _pi(ic), C.byref(IJ),
+                          C.byref(N1), C.byref(N2), C.byref(N3), C.byref(IP), C.byref(INFO),
+                          C.byref(GAP))
+        j = IJ.value
+        if j not in counts:
+            break
+        counts[j] += 1
+        a1, a2, a3 = N1.value - 1, N2.value - 1, N3.value - 1
+        xi = x[a1:a1 + n].copy()
+        y = np.empty(n, dtype=np.complex128)
+        if j in (1, 3):
+            matvec_a(xi, y)
+            x[a2:a2 + n] = y
+        if j in (1, 4):
+            matvec_b(xi, y)
+            x[a3:a3 + n] = y
+    r = _result(x, n, eigs, res, NE, IT, INFO, GAP, SH, DT)
+    r.ijob_counts = counts
+    return r
+
+
 def zjdrevcom_gep_host(n, matvec_a, matvec_b, psolve, *, neig, isearch=0, sigma=0.0, ninit=0,
                        madspace=20, maxiter=1000, tol=1e-10, lx=None, iprint=0):
     """ZJDREVCOM_GEP through `zjdrevcom_gep_`: IJOB 1 (A and B products),

@@ -267,3 +267,28 @@ def test_magnetic_anderson_interior():
     assert np.allclose(Z.conj().T @ Z, np.eye(6), atol=1e-8)
     true_res = np.linalg.norm(H @ Z - Z * r.eigs, axis=0)
     assert np.all(true_res <= 5 * TOL), true_res
+
+
+def test_zpjdrevcom_gep_smallest_with_callers_products():
+    """ZPJDREVCOM_GEP on the pencil of samples/EXAMPLE1_GEP_double_complex.f:
+    the preconditioner from the matrices, the products from the caller (IJOB
+    1, 3 and 4), the 5 smallest eigenvalues against scipy.linalg.eigh(H, M)."""
+    from jadamilu_amd import zjd
+    n = 400
+    ia, ja, a = tri_example(n)
+    b = np.array([1.0 if ja[k] == i + 1 else (0.25 if (i + 1) % 2 == 0 else 0.25j)
+                  for i in range(n) for k in range(ia[i] - 1, ia[i + 1] - 1)],
+                 dtype=np.complex128)
+    H = zjd.hermitian_dense(n, a, ja, ia)
+    Mm = zjd.hermitian_dense(n, b, ja, ia)
+    lam = sla.eigh(H, Mm, eigvals_only=True)
+    r = zjd.zpjdrevcom_gep_host(n, a, ja, ia, b, ja, ia, lambda x, y: np.copyto(y, H @ x),
+                                lambda x, y: np.copyto(y, Mm @ x), neig=5, isearch=0,
+                                madspace=20, maxiter=2000, tol=TOL, droptol=1e-3, mem=20.0)
+    assert r.info == 0 and r.neig == 5
+    assert np.allclose(np.sort(r.eigs), lam[:5], rtol=0, atol=1e-9)
+    assert r.ijob_counts[1] > 0
+    Z = r.x
+    res = np.linalg.norm(H @ Z - (Mm @ Z) * r.eigs, axis=0) / np.linalg.norm(Mm @ Z, axis=0)
+    assert np.allclose(r.res, res, rtol=1e-3, atol=1e-13)
+    assert np.all(res <= 5 * TOL), res
