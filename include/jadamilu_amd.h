@@ -351,7 +351,9 @@ int jdhip_tall_gemm(jdhip_ctx *ctx, long n, int k, int j, const double *v_dev, l
  * level factored densely), "spmv_lanes", "ildl_graph" (1: replay the
  * preconditioner apply from captured HIP graphs; default 0), "ildl_storage",
  * "ildl_xcd", "ildl_sigma" (device layout of the factor), "ildl_threads"
- * (host threads of the factorization), and for ICNTL(5) != 0: "ildl_vec32"
+ * (host threads of the factorization), "sync_blocking" (1: the iteration's
+ * waits for the device sleep on a blocking-sync event instead of spinning;
+ * default 0), and for ICNTL(5) != 0: "ildl_vec32"
  * (work vectors of the apply: 1 FP32, 0 FP64, -1 FP32 for definite factors
  * only, the default), "inner_fp32" (1 = FP32 inner solver; default 0: FP64
  * inner solver as in the reference), "inner_spmv64" (FP32 inner solver: 1 =

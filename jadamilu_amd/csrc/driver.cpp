@@ -1255,6 +1255,7 @@ int jdhip_set_option(jdhip_ctx *ctx, const char *name, double value) {
     else if (k == "ildl_xcd") S.ildl_xcd = (int)value;
     else if (k == "ildl_sigma") S.ildl_sigma = (int)value;
     else if (k == "ildl_threads") S.ildl_threads = (int)value;
+    else if (k == "sync_blocking") S.ctx.blocking_sync = value != 0.0;
     else if (k == "ildl_vec32") S.ildl_vec32 = (int)value;
     else if (k == "ildl_autotune") S.ildl_autotune = value != 0.0;
     else if (k == "ildl_tune_tol") S.ildl_tune_tol = value;

@@ -40,6 +40,7 @@ void DevCtx::init(int dev) {
     std::memset(h_sc, 0, sizeof(InnerScalars));
     JDHIP_CHECK(hipMalloc(&d_gram, sizeof(double) * kGramScratch));
     JDHIP_CHECK(hipMalloc(&d_gout, sizeof(double) * kMaxCols * kMaxCols));
+    JDHIP_CHECK(hipEventCreateWithFlags(&ev_sync, hipEventBlockingSync | hipEventDisableTiming));
 }
 
 void DevCtx::release() {
@@ -55,6 +56,8 @@ void DevCtx::release() {
     hipFree(d_gout);
     hipHostFree(h_pin);
     hipHostFree(h_sc);
+    if (ev_sync) hipEventDestroy(ev_sync);
+    ev_sync = nullptr;
     hipStreamDestroy(stream);
     stream = nullptr;
     d_tmp = nullptr;

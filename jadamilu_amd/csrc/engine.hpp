@@ -48,7 +48,19 @@ struct DevCtx {
     void init(int dev);
     void release();
     void ensure_tmp(long len);
-    void sync() { JDHIP_CHECK(hipStreamSynchronize(stream)); }
+    // the host's waits for the stream: spinning (hipStreamSynchronize, the
+    // default) or, with blocking_sync, sleeping on an event created with
+    // hipEventBlockingSync, which leaves the core to other host threads
+    hipEvent_t ev_sync = nullptr;
+    bool blocking_sync = false;
+    void sync() {
+        if (blocking_sync) {
+            JDHIP_CHECK(hipEventRecord(ev_sync, stream));
+            JDHIP_CHECK(hipEventSynchronize(ev_sync));
+        } else {
+            JDHIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
     // device slots
     double *slot(int k) { return d_small + (long)k * kMaxCols; }
 };

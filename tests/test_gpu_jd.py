@@ -215,3 +215,33 @@ def test_input_errors_device():
                      (dict(neig=2, lx=10), -10)]:
         r = jd.pjdrevcom_host(10, np.ones(10), [-1], [-1], mv, **kw)
         assert r.info == code, (kw, r.info)
+
+
+def test_blocking_sync_same_result():
+    """sync_blocking changes only how the host waits for the stream: the
+    same kernels run in the same order, so eigenvalues, residuals and the
+    MATVEC count are identical."""
+    from jadamilu_amd.jd import Context, lx_optimal
+    dev = _dev()
+    ia, ja, a = ex9()
+    n = ia.size - 1
+    diag = np.arange(1, n + 1, dtype=float)
+    out = []
+    for blocking in (0, 1):
+        ctx = Context(0)
+        ctx.set_option("sync_blocking", blocking)
+        ctx.set_matrix(ia, ja, a)
+        lx = lx_optimal(n, 5, 20)
+        X = torch.zeros(lx, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        r = ctx.pjdrevcom(n, diag, [-1], [-1], X.data_ptr(), lx, neig=5, isearch=0,
+                          sigma=0.0, madspace=20, maxiter=20000, tol=1e-10,
+                          icntl=(0, 0, 0, 0, 0), ninit=0)
+        torch.cuda.synchronize()
+        out.append((r.info, r.iter, np.array(r.eigs), np.array(r.res),
+                    X[: n * r.neig].cpu().numpy()))
+        ctx.close()
+    (i0, it0, e0, r0, x0), (i1, it1, e1, r1, x1) = out
+    assert i0 == 0 and i1 == 0
+    assert it0 == it1
+    assert np.array_equal(e0, e1) and np.array_equal(r0, r1) and np.array_equal(x0, x1)
