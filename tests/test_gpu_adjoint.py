@@ -84,6 +84,29 @@ def test_csr_gradient_dense():
     np.testing.assert_allclose(g, gref, rtol=0, atol=1e-9 * np.abs(gref).max())
 
 
+def test_csr_gradient_dense_l12():
+    """dL/dA over the whole CSR pattern at n = 1,728 (L = 12, 4 eigenvalues
+    nearest 0, multilevel ILDL preconditioner inside the solve): every one
+    of the 6,912 stored entries against the dense reference
+    w_k x_k(i) x_k(j) (doubled off the diagonal)."""
+    from jadamilu_amd.autograd import eigvals
+    dev = _dev()
+    L, neig = 12, 4
+    ia, ja, a = M.anderson(L, 15.0, 11)
+    wts = np.array([0.9, -0.6, 1.4, -1.1])
+    vals = torch.tensor(a, device=dev, requires_grad=True)
+    lam = eigvals(ia, ja, vals, neig=neig, tol=1e-13)
+    (torch.tensor(wts, device=dev) * lam).sum().backward()
+    g = vals.grad.cpu().numpy()
+    _, X = _dense_ref(ia, ja, a, lam.detach().cpu().numpy())
+    n = ia.size - 1
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    cols = ja - 1
+    gref = ((X[rows] * X[cols]) @ wts) * np.where(rows == cols, 1.0, 2.0)
+    assert g.size == ia[-1] - 1
+    np.testing.assert_allclose(g, gref, rtol=0, atol=1e-9 * np.abs(gref).max())
+
+
 @pytest.mark.slow
 def test_anderson40_potential_gradient_fd64():
     """The differentiation workload at full size: Anderson L=40 (n=64,000),

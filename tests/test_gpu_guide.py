@@ -139,7 +139,8 @@ def test_dpjd_example11_two_stages(golden):
     assert r1.info == 0 and r1.neig == 5
     np.testing.assert_allclose(np.sort(r1.eigs), np.sort([t["eig"] for t in g1["table"]]),
                                rtol=1e-6)
-    assert abs(r1.iter - g1["matvec"]) <= 3, (r1.iter, g1["matvec"])
+    # an exact factor (DROPTOL 0): the count is the guide's to the MATVEC
+    assert r1.iter == g1["matvec"], (r1.iter, g1["matvec"])
     assert st1["n_precond_setups"] == 1
     assert st1["setup_shifts"][0] == g1["precond_setups"][0]["shift"]
     assert abs(st1["setup_fills"][0] - g1["precond_setups"][0]["fill"]) <= 0.005
@@ -152,7 +153,7 @@ def test_dpjd_example11_two_stages(golden):
     assert r2.info == 0 and r2.neig == 5
     np.testing.assert_allclose(np.sort(r2.eigs), np.sort([t["eig"] for t in g2["table"]]),
                                rtol=1e-10)
-    assert abs(r2.iter - g2["matvec"]) <= 3, (r2.iter, g2["matvec"])
+    assert r2.iter == g2["matvec"], (r2.iter, g2["matvec"])
     assert st2["n_precond_setups"] == 0  # the factor of the first call is reused
 
 
