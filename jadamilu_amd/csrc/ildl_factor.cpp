@@ -163,19 +163,19 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     std::vector<T> av(A.idx.size());
     for (int k = 0; k < n; ++k) ap[k + 1] = ap[k] + (A.ptr[order[k] + 1] - A.ptr[order[k]]);
     {
-        std::vector<std::pair<int, T>> row;
+        // A's pattern and values are symmetric, so entry A(o, j) of row
+        // o = order[k] is also entry (c, k) of the permuted row c = pos[j]:
+        // visiting the rows k in increasing order fills every permuted row in
+        // increasing column order, with no sort.  The value is formed as
+        // s[row] * a * s[column] of the permuted row's own old indices.
+        std::vector<int> fill(ap.begin(), ap.end() - 1);
         for (int k = 0; k < n; ++k) {
-            int o = order[k];
-            row.clear();
+            const int o = order[k];
             for (int t = A.ptr[o]; t < A.ptr[o + 1]; ++t) {
-                int j = A.idx[t];
-                row.emplace_back(pos[j], (T)(s[o] * A.val[t] * s[j]));
-            }
-            std::sort(row.begin(), row.end(),
-                      [](const auto &x, const auto &y) { return x.first < y.first; });
-            for (size_t t = 0; t < row.size(); ++t) {
-                ai[ap[k] + t] = row[t].first;
-                av[ap[k] + t] = row[t].second;
+                const int j = A.idx[t];
+                const int f = fill[pos[j]]++;
+                ai[f] = k;
+                av[f] = (T)(s[j] * A.val[t] * s[o]);
             }
         }
     }
