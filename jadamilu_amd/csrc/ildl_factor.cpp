@@ -202,10 +202,11 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     cval.reserve(ai.size() * 2);
     // row lists of L: rows -> (column, value)
     std::vector<long> rhead(n, -1), rtail(n, -1), rnext;
-    std::vector<int> rcol;
+    std::vector<int> rcol, rrow;  // column and row of every stored entry
     std::vector<T> rval;
     rnext.reserve(ai.size() * 2);
     rcol.reserve(ai.size() * 2);
+    rrow.reserve(ai.size() * 2);
     rval.reserve(ai.size() * 2);
     // sparse accumulators (branch-free insertion: the index is always
     // written at the end of the list and the list grows only on first touch)
@@ -308,6 +309,7 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             // append to row list of kv.first (keeps column order increasing)
             rnext.push_back(-1);
             rcol.push_back(k);
+            rrow.push_back(kv.first);
             rval.push_back(kv.second);
             const long id = (long)rcol.size() - 1;
             if (rtail[kv.first] < 0) rhead[kv.first] = id;
@@ -653,13 +655,14 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             L.dpair[bpos[k]] = bpos[partner[k]];
         }
     }
-    // rows of L_B and L_CB from the row lists, whose entries are in column
-    // (acceptance) order, so column indices ascend within each row
-    for (int r = 0; r < n; ++r) {
-        int c = 0;
-        for (long e = rhead[r]; e >= 0; e = rnext[e]) ++c;
-        if (status[r] == 1) lbcnt[bpos[r] + 1] = c;
-        else lccnt[cpos[r] + 1] = c;
+    // rows of L_B and L_CB: the stored entries are numbered in column
+    // (acceptance) order, so one counting pass by row keeps the column
+    // indices of every row ascending (the order of its row list)
+    const long nent = (long)rcol.size();
+    for (long e = 0; e < nent; ++e) {
+        const int r = rrow[e];
+        if (status[r] == 1) ++lbcnt[bpos[r] + 1];
+        else ++lccnt[cpos[r] + 1];
     }
     for (int i = 0; i < nB; ++i) lbcnt[i + 1] += lbcnt[i];
     for (int i = 0; i < nC; ++i) lccnt[i + 1] += lccnt[i];
@@ -671,15 +674,17 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
     L.lc_val.assign(lccnt[nC], T(0));
     // L_CB by columns for the Schur complement
     std::vector<int> lcc_ptr(nB + 1, 0);
-    for (int r = 0; r < n; ++r) {
-        const bool inB = status[r] == 1;
-        int f = inB ? lbcnt[bpos[r]] : lccnt[cpos[r]];
-        for (long e = rhead[r]; e >= 0; e = rnext[e], ++f) {
+    {
+        std::vector<int> fb(lbcnt.begin(), lbcnt.end() - 1), fc(lccnt.begin(), lccnt.end() - 1);
+        for (long e = 0; e < nent; ++e) {
+            const int r = rrow[e];
             const int b = bpos[rcol[e]];
-            if (inB) {
+            if (status[r] == 1) {
+                const int f = fb[bpos[r]]++;
                 L.lb_idx[f] = b;
                 L.lb_val[f] = rval[e];
             } else {
+                const int f = fc[cpos[r]]++;
                 L.lc_idx[f] = b;
                 L.lc_val[f] = rval[e];
                 lcc_ptr[b + 1]++;
