@@ -173,7 +173,9 @@ void factor_level(const SymCsr &A, const IldlParams &prm, double condest, long &
             const int o = order[k];
             for (int t = A.ptr[o]; t < A.ptr[o + 1]; ++t) {
                 const int j = A.idx[t];
-                const int f = fill[pos[j]]++;
+                const int c = pos[j];
+                const int f = fill[c]++;
+                if (f >= ap[c + 1]) throw std::runtime_error("ildl: level matrix pattern is not symmetric");
                 ai[f] = k;
                 av[f] = (T)(s[j] * A.val[t] * s[o]);
             }
